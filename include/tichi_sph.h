@@ -1,0 +1,265 @@
+/*
+ * tichi_sph.h - C ABI of the MI355X (gfx950) implementation of Tichi_SPH's
+ * per-step particle hot path.
+ *
+ * The library is a set of hand-written HIP kernels behind plain C entry
+ * points.  Every pointer inside TsphObject / TsphControl / workspace is a
+ * DEVICE pointer owned by the caller (in the Python host layer: torch
+ * tensors); the library never allocates or frees device memory and never
+ * synchronises the host.  All entry points enqueue work on `stream`
+ * (a hipStream_t passed as void*) and return 0 on success or a negative
+ * TSPH_E* code; tsph_last_error() then holds a message (thread-local).
+ *
+ * Arithmetic type: TsphParams.precision selects float (4) or double (8) for
+ * every floating-point array; integer arrays are always int32.
+ *
+ * Layout (per particle object = one reference `Fluid`, sph_obj.py:8-93):
+ *   vector fields are 4-wide: [capacity][4] = (x, y, z, w); z = 0 in 2D.
+ *   pos.w holds the solver alias X (mass for DFSPH, rest_volume for VFSPH,
+ *   sph_obj.py:65-73) so one 16-byte load fetches position and X.
+ *   volume_frac is [capacity][phase_num]; drift_vel is [capacity][phase_num][4].
+ *   The fused step keeps particles in cell order (a counting sort each step);
+ *   `pid` holds each slot's reference particle index (creation order).
+ */
+#ifndef TICHI_SPH_H
+#define TICHI_SPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TSPH_ABI_VERSION 1
+#define TSPH_MAX_PHASE 8
+#define TSPH_MAX_ITER_LOG 256
+
+enum {
+  TSPH_OK = 0,
+  TSPH_EINVAL = -1,   /* bad argument / shape */
+  TSPH_ECAPACITY = -2,/* capacity or workspace too small */
+  TSPH_EHIP = -3,     /* HIP runtime error (message has the HIP string) */
+  TSPH_EUNSUPPORTED = -4
+};
+
+/* Solver types, sph_config.py:79 / sph_obj.py:65-73. */
+enum { TSPH_DFSPH = 0, TSPH_VFSPH = 1 };
+
+/*
+ * Per-scene constants.  Replaces the reference's `Config` singleton
+ * (sph_config.py:83-232): the host layer fills it from config.json and the
+ * scenario's sim_env with the same f64->f32 rounding points.
+ */
+typedef struct TsphParams {
+  int32_t dim, phase_num, solver_type, precision;
+  int32_t neighb_range, ntemp, node_num, list_cap_f;
+  int32_t grid_size[4], coder[4];
+  int32_t cs, wc_gamma, iter_min, iter_max;
+  int32_t list_cap_b, flags, pad0, pad1;
+  double lb[4], rt[4], gravity[4];
+  double phase_rest_density[TSPH_MAX_PHASE];
+  double part_size[5], kernel_h[5], kernel_sig[4];
+  double r2_max; /* largest r^2 whose fl(fl(sqrt(r^2))/h) < 1 in this precision */
+  double dynamic_viscosity, surface_tension_gamma, fbm_diffusion, fbm_convection, cfl_factor;
+  double divergence_threshold, compression_threshold;
+} TsphParams;
+
+/*
+ * Device pointers of one particle object (the reference `Fluid`,
+ * sph_obj.py:8-93).  Names follow the reference attributes.
+ */
+typedef struct TsphObject {
+  int32_t n, capacity, uid, pad;
+  void *pos, *vel, *vel_adv, *acce_adv, *acce, *normal, *alpha_1;      /* T[cap][4] */
+  void *mass, *rest_density, *rest_volume;                             /* T[cap] */
+  void *volume_frac, *volume_frac_tmp;                                 /* T[cap][P] */
+  void *drift_vel;                                                     /* T[cap][P][4] */
+  void *W, *sph_compression, *sph_density, *psi_adv, *alpha, *alpha_2; /* T[cap] */
+  void *kappa, *vol, *fbm_zeta, *pressure;                             /* T[cap] */
+  int32_t *pid, *flag, *cell;                 /* [cap]: reference index, FBM flag, cell key */
+  int32_t *cell_structured;                   /* [cap][4] neighb_cell_structured_seq */
+  int32_t *in_cell;                           /* [cap] neighb_in_cell_seq */
+  int32_t *cell_count, *cell_start;           /* [node_num + 2] per-object cell table */
+  int32_t *nbr_f, *nbr_b;                     /* [list_cap][cap] slot-major neighbour lists */
+  int32_t *nbr_nf, *nbr_nb;                   /* [cap] list lengths (cap+1 = overflow) */
+  /* back buffers of the persistent state, swapped by tsph_sort_object */
+  void *pos_b, *vel_b, *mass_b, *rest_density_b, *rest_volume_b, *volume_frac_b;
+  int32_t *pid_b;
+} TsphObject;
+
+/*
+ * Device-resident control block: the reference's 0-D fields
+ * (config.dt, fluid.compression, statistics_*, general_flag) plus the
+ * solver's device-side stop state.  Allocate tsph_control_bytes() bytes of
+ * device memory, zero it once, then pass it to every call.
+ */
+typedef struct TsphControl {
+  double dt;             /* config.dt (sph.py:41-47) */
+  double compression;    /* fluid.compression (sph.py:219-226); persists across solves */
+  int32_t it, stop, counter, general_flag;
+  int32_t div_iter, incom_iter, fbm_rounds, solve;
+  double ke, pe;         /* statistics_kinetic/gravity_potential_energy (sph.py:272-279) */
+  int32_t run_u[TSPH_MAX_ITER_LOG];
+  double err_div[TSPH_MAX_ITER_LOG];
+  double err_incom[TSPH_MAX_ITER_LOG];
+  int32_t overflow_f, overflow_b, counter2, pad;
+} TsphControl;
+
+const char* tsph_last_error(void);
+int tsph_abi_version(void);
+size_t tsph_control_bytes(void);
+
+/* Workspace needed by the calls below for an object of `capacity` slots. */
+size_t tsph_workspace_bytes(const TsphParams* p, int32_t capacity);
+
+/* ----------------------------------------------------------------------
+ * Neighbour grid
+ * ---------------------------------------------------------------------- */
+
+/*
+ * Replaces Ngrid.clear_node/encode/mem_shift/fill_node (sph_obj.py:295-323)
+ * for the fused step: computes every particle's cell key (node_encode +
+ * dim_encode, sph_util.py:65-72), counts particles per cell, prefix-sums the
+ * counts and moves the particles into cell order with ascending `pid`
+ * inside each cell (the reference's single-thread bucket order).  Particles
+ * with an invalid code (0 < code < node_num fails, sph_obj.py:305) go to a
+ * trailing bucket.  Swaps the object's front/back state pointers in *obj.
+ * When `cfl` is non-zero it also performs cfl_condition (sph.py:41-47) into
+ * ctrl->dt from the object's velocities.
+ */
+int tsph_sort_object(const TsphParams* p, TsphObject* obj, TsphControl* ctrl, int32_t cfl,
+                     void* workspace, size_t ws_bytes, void* stream);
+
+/*
+ * The reference's shared Ngrid arrays (sph_obj.py:283-293), built over the
+ * objects' current storage order: node_part_count/shift/shift_count per cell
+ * and part_pid_in_node/part_uid_in_node per slot, fluid entries before
+ * boundary entries in a cell and ascending reference index within each
+ * (the single-thread order of main.py:178-183).  Also writes each object's
+ * neighb_cell_seq (cell), neighb_cell_structured_seq and neighb_in_cell_seq.
+ * part_pid_in_node holds storage slots; objs[u] must have uid u (1 or 2
+ * objects).  Used by the reference-surface
+ * kernels (tsph_ref_kernel) and for parity export.
+ */
+int tsph_ngrid_build(const TsphParams* p, TsphObject* const* objs, int32_t nobj,
+                     int32_t* node_part_count, int32_t* node_part_shift,
+                     int32_t* node_part_shift_count, int32_t* part_pid_in_node,
+                     int32_t* part_uid_in_node, void* workspace, size_t ws_bytes, void* stream);
+
+/*
+ * NeighborSearch.fill_neighbs (NeighborSearch.py:84-101) over a built Ngrid:
+ * for each particle of `obj`, the unfiltered candidate list (self included)
+ * grouped by uid, template order then cell order.  Pass neighbs == NULL to
+ * get only neighbs_count (per particle, per uid) from which the caller sizes
+ * neighbs and computes row starts; then call again with neighbs and
+ * row_start (int32[n+1]).  neighbs_shift/count are [n][n_uids].
+ */
+int tsph_neighbor_lists(const TsphParams* p, const TsphObject* obj, int32_t n_uids,
+                        const int32_t* node_part_count, const int32_t* node_part_shift,
+                        const int32_t* part_pid_in_node, const int32_t* part_uid_in_node,
+                        const int32_t* row_start, int32_t* neighbs, int32_t* neighbs_shift,
+                        int32_t* neighbs_count, void* stream);
+
+/* ----------------------------------------------------------------------
+ * Fused per-step kernels (sph_step, main.py:175-256).  Objects must have
+ * been sorted by tsph_sort_object in this step.
+ * ---------------------------------------------------------------------- */
+
+/*
+ * SPH_clean_value + SPH_prepare_attr (f,f)(f,b)(b,b)(b,f) + SPH_prepare_alpha_1
+ * (f,f)(f,b) + SPH_prepare_alpha_2 (f,f)(b,f) + SPH_prepare_alpha (f)(b) +
+ * SPH_vel_2_vel_adv (sph.py:22-105, 253-256; main.py:185-200), in one sweep
+ * per particle.  Records each fluid particle's in-range fluid and boundary
+ * neighbours (0 < r < h) in reference order for the later sweeps, and opens
+ * the divergence solve's stop state.
+ */
+int tsph_density_alpha(const TsphParams* p, TsphObject* fluid, TsphObject* bound,
+                       TsphControl* ctrl, void* workspace, size_t ws_bytes, void* stream);
+
+/*
+ * Opens a Jacobi solve (main.py:199-201 / 221-222): it = 0 and the entry test
+ * of the while loop.  solve = 0 divergence, 1 density.
+ */
+int tsph_solver_begin(const TsphParams* p, TsphControl* ctrl, int32_t solve, void* stream);
+
+/*
+ * One predicated Jacobi iteration k of a solve: IPPE_adv_psi_init +
+ * IPPE_adv_psi (f,f)(f,b) + IPPE_psi_adv_non_negative (sph.py:196-226) with a
+ * deterministic device reduction of the compression, then
+ * IPPE_update_vel_adv (f,f)(f,b) (sph.py:229-244).  Does nothing once the
+ * device stop state says the loop has ended (main.py:201-213, 222-234).
+ */
+int tsph_solver_iteration(const TsphParams* p, TsphObject* fluid, TsphObject* bound,
+                          TsphControl* ctrl, int32_t k, void* workspace, size_t ws_bytes,
+                          void* stream);
+
+/*
+ * SPH_vel_adv_2_vel + SPH_advection_gravity_acc + SPH_advection_viscosity_acc
+ * (f,f) + SPH_advection_surface_tension_acc (f,f) + SPH_advection_update_vel_adv
+ * (sph.py:108-166, 247-262; main.py:214-219).
+ */
+int tsph_advect(const TsphParams* p, TsphObject* fluid, TsphObject* bound, TsphControl* ctrl,
+                void* workspace, size_t ws_bytes, void* stream);
+
+/*
+ * One FBM round (main.py:244-248): SPH_FBM_clean_tmp + SPH_FBM_convect (f,f) +
+ * SPH_FBM_diffuse (f,f) + SPH_FBM_check_tmp (sph.py:301-372).  Predicated on
+ * ctrl->general_flag; ctrl->fbm_rounds counts rounds that ran.
+ */
+int tsph_fbm_round(const TsphParams* p, TsphObject* fluid, TsphControl* ctrl, void* workspace,
+                   size_t ws_bytes, void* stream);
+
+/*
+ * Step tail: [SPH_update_volume_frac] + SPH_update_mass + SPH_update_pos +
+ * SPH_update_energy (sph.py:265-286, 375-379; main.py:250-253).  fbm != 0
+ * applies the volume-fraction update first.
+ */
+int tsph_tail(const TsphParams* p, TsphObject* fluid, TsphControl* ctrl, int32_t fbm,
+              void* workspace, size_t ws_bytes, void* stream);
+
+/* ----------------------------------------------------------------------
+ * Reference kernel surface: the sph.py kernels one by one, with the
+ * reference's (ngrid, obj, nobj) signature, walking the Ngrid built by
+ * tsph_ngrid_build exactly as the reference does.  `kernel` is one of the
+ * TSPH_K_* ids; nobj may be NULL for per-particle kernels.
+ * ---------------------------------------------------------------------- */
+enum {
+  TSPH_K_SPH_clean_value = 0,                 /* sph.py:22-38 */
+  TSPH_K_cfl_condition = 1,                   /* sph.py:41-47 */
+  TSPH_K_SPH_prepare_attr = 2,                /* sph.py:50-64 */
+  TSPH_K_SPH_prepare_alpha_1 = 3,             /* sph.py:67-81 */
+  TSPH_K_SPH_prepare_alpha_2 = 4,             /* sph.py:84-97 */
+  TSPH_K_SPH_prepare_alpha = 5,               /* sph.py:100-105 */
+  TSPH_K_SPH_advection_gravity_acc = 6,       /* sph.py:108-111 */
+  TSPH_K_SPH_advection_viscosity_acc = 7,     /* sph.py:114-129 */
+  TSPH_K_SPH_advection_surface_tension_acc = 8, /* sph.py:132-166 */
+  TSPH_K_WC_pressure_val = 9,                 /* sph.py:169-175 */
+  TSPH_K_WC_pressure_acce = 10,               /* sph.py:178-193 */
+  TSPH_K_IPPE_adv_psi_init = 11,              /* sph.py:196-199 */
+  TSPH_K_IPPE_adv_psi = 12,                   /* sph.py:202-216 */
+  TSPH_K_IPPE_psi_adv_non_negative = 13,      /* sph.py:219-226 */
+  TSPH_K_IPPE_update_vel_adv = 14,            /* sph.py:229-244 */
+  TSPH_K_SPH_advection_update_vel_adv = 15,   /* sph.py:247-250 */
+  TSPH_K_SPH_vel_2_vel_adv = 16,              /* sph.py:253-256 */
+  TSPH_K_SPH_vel_adv_2_vel = 17,              /* sph.py:259-262 */
+  TSPH_K_SPH_update_pos = 18,                 /* sph.py:265-269 */
+  TSPH_K_SPH_update_energy = 19,              /* sph.py:272-279 */
+  TSPH_K_SPH_update_mass = 20,                /* sph.py:282-286 */
+  TSPH_K_SPH_FBM_clean_tmp = 21,              /* sph.py:301-305 */
+  TSPH_K_SPH_FBM_diffuse = 22,                /* sph.py:308-326 */
+  TSPH_K_SPH_FBM_convect = 23,                /* sph.py:329-363 */
+  TSPH_K_SPH_FBM_check_tmp = 24,              /* sph.py:366-372 */
+  TSPH_K_SPH_update_volume_frac = 25,         /* sph.py:375-379 */
+  TSPH_K_COUNT = 26
+};
+
+int tsph_ref_kernel(const TsphParams* p, int32_t kernel, TsphObject* obj, const TsphObject* nobj,
+                    const int32_t* node_part_count, const int32_t* node_part_shift,
+                    const int32_t* part_pid_in_node, const int32_t* part_uid_in_node,
+                    TsphControl* ctrl, void* workspace, size_t ws_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TICHI_SPH_H */

@@ -1,0 +1,249 @@
+"""GPU parity: the HIP path against the CPU oracle on the same inputs.
+
+Integer outputs (cell keys, sort order, grid arrays, neighbour sets) must be
+bit-exact.  Floating-point fields must satisfy
+    |gpu - oracle| <= tol * max(|oracle|, RMS(oracle field))
+with tol = 1e-5 for FP32, plus equal solver iteration
+counts.  The error of both against the FP64 oracle is printed.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from conftest import load_scene
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if b.size == 0:
+        return 0.0
+    rms = math.sqrt(float((b ** 2).mean()))
+    den = np.maximum(np.abs(b), rms)
+    den = np.where(den == 0, 1.0, den)
+    return float((np.abs(a - b) / den).max())
+
+
+def make(name, precision=4, **kw):
+    from tichi_sph_amd import simulation_from_dicts
+    cb, sb = load_scene(name)
+    print(f"[case] {name} precision={precision}", flush=True)
+    sim = simulation_from_dicts(cb, sb, precision=precision, **kw)
+    o = O.Oracle(cb, sb, precision)
+    o.init_scenario()
+    return sim, o
+
+
+FIELDS_F = ["pos", "vel", "vel_adv", "sph_density", "sph_compression", "alpha", "psi_adv", "W",
+            "acce_adv", "mass", "rest_density"]
+
+
+def compare_fields(sim, o, fields, tol=TOL, obj=0, label=""):
+    fl = sim.fluid if obj == 0 else sim.bound
+    worst = {}
+    for f in fields:
+        g = fl.to_numpy(f)
+        r = o.get(obj, f)
+        e = rel_err(g, r)
+        worst[f] = e
+        exact = float(np.mean(g.astype(np.float64) == r.astype(np.float64)))
+        print(f"  {label} obj{obj} {f:16s} rel_err={e:.2e} bit_exact_frac={exact:.4f}", flush=True)
+    bad = {f: e for f, e in worst.items() if e > tol}
+    assert not bad, bad
+    return worst
+
+
+def test_initial_state_bit_exact():
+    sim, o = make("dambreak_3d_8k")
+    for obj, fl in ((0, sim.fluid), (1, sim.bound)):
+        assert fl.part_num == o.n[obj]
+        for f in ("pos", "vel", "mass", "rest_density", "rest_volume", "volume_frac"):
+            g, r = fl.to_numpy(f), o.get(obj, f)
+            assert np.array_equal(g.reshape(r.shape), r), f
+
+
+@pytest.mark.parametrize("name", ["dambreak_2d_8k", "dambreak_3d_8k"])
+def test_reference_grid_bit_exact(name):
+    sim, o = make(name)
+    sim.ngrid.clear_node(); sim.ngrid.encode(sim.fluid); sim.ngrid.encode(sim.bound)
+    sim.ngrid.mem_shift(); sim.ngrid.fill_node(sim.fluid); sim.ngrid.fill_node(sim.bound)
+    for k in ("clear_node",):
+        o.kernel(k, 0)
+    o.kernel("encode", 0); o.kernel("encode", 1); o.kernel("mem_shift", 0)
+    o.kernel("fill_node", 0); o.kernel("fill_node", 1)
+    torch.cuda.synchronize()
+    nn = o.c.node_num
+    for f in ("node_part_count", "node_part_shift", "node_part_shift_count"):
+        assert np.array_equal(getattr(sim.ngrid, f).cpu().numpy(), o.grid(f, nn)), f
+    tot = int(o.grid("node_part_count", nn).sum())
+    uid = sim.ngrid.part_uid_in_node[:tot].cpu().numpy()
+    slot = sim.ngrid.part_pid_in_node[:tot].cpu().numpy()
+    pid = np.where(uid == 0, sim.fluid.t["pid"].cpu().numpy()[np.where(uid == 0, slot, 0)],
+                   sim.bound.t["pid"].cpu().numpy()[np.where(uid == 1, slot, 0)])
+    assert np.array_equal(uid, o.grid("part_uid_in_node", tot))
+    assert np.array_equal(pid, o.grid("part_pid_in_node", tot))
+    for obj, fl in ((0, sim.fluid), (1, sim.bound)):
+        assert np.array_equal(fl.to_numpy("neighb_cell_seq"), o.get(obj, "neighb_cell_seq"))
+        assert np.array_equal(fl.to_numpy("neighb_cell_structured_seq"), o.get(obj, "neighb_cell_structured_seq"))
+        assert np.array_equal(fl.to_numpy("neighb_in_cell_seq"), o.get(obj, "neighb_in_cell_seq"))
+
+
+def test_neighbor_search_lists_bit_exact():
+    from tichi_sph_amd import NeighborSearch
+    sim, o = make("dambreak_2d_8k")
+    ns = NeighborSearch(sim, 2)
+    ns.establish_neighbs(sim.fluid, sim.bound)
+    o.kernel("establish_neighbs", 0)
+    L = O.lib()
+    for obj, fl in ((0, sim.fluid), (1, sim.bound)):
+        n = o.n[obj]
+        tot = L.oracle_ns_total(o.h, obj)
+        nb = np.zeros(tot, np.int32); st = np.zeros(n + 1, np.int32)
+        sh = np.zeros(2 * n, np.int32); ct = np.zeros(2 * n, np.int32)
+        L.oracle_ns_get(o.h, obj, nb.ctypes.data, st.ctypes.data, sh.ctypes.data, ct.ctypes.data)
+        part = ns.parts[obj]
+        pidf = sim.fluid.t["pid"].cpu().numpy()
+        pidb = sim.bound.t["pid"].cpu().numpy()
+        gshift = part.neighbs_shift.cpu().numpy().reshape(-1)
+        gcount = part.neighbs_count.cpu().numpy().reshape(-1)
+        grow = part.row_start.cpu().numpy()
+        gnb = part.neighbs.cpu().numpy()
+        mypid = fl.t["pid"].cpu().numpy()
+        inv = np.empty(n, np.int64)
+        inv[mypid[:n]] = np.arange(n)
+        assert np.array_equal(gshift.reshape(n, 2)[inv].reshape(-1), sh)
+        assert np.array_equal(gcount.reshape(n, 2)[inv].reshape(-1), ct)
+        for i in range(n):
+            s = inv[i]
+            row = gnb[grow[s]: grow[s + 1]]
+            nf = gcount[2 * s]
+            mapped = np.concatenate([pidf[row[:nf]], pidb[row[nf:]]])
+            assert np.array_equal(mapped, nb[st[i]: st[i + 1]]), i
+
+
+def test_fused_sort_is_canonical_cell_order():
+    sim, o = make("dambreak_3d_8k")
+    sim.sph_step()
+    torch.cuda.synchronize()
+    f = sim.fluid
+    n = f.part_num
+    key = f.t["cell"][:n].cpu().numpy()
+    pid = f.t["pid"][:n].cpu().numpy()
+    assert np.all(np.diff(key) >= 0)
+    same = np.diff(key) == 0
+    assert np.all(np.diff(pid)[same] > 0)
+    assert np.array_equal(np.sort(pid), np.arange(n))
+    start = f.t["cell_start"].cpu().numpy()
+    cnt = f.t["cell_count"].cpu().numpy()
+    nn = o.c.node_num
+    assert start[nn + 1] == n
+    assert np.array_equal(np.diff(start[: nn + 2]), cnt[: nn + 1])
+
+
+@pytest.mark.parametrize("name", ["dambreak_2d_8k", "dambreak_3d_8k"])
+def test_reference_surface_step_matches_oracle(name):
+    sim, o = make(name)
+    di, ii, derr, ierr, _ = sim.sph_step_reference()
+    st = o.step(O.STEP_CFL)
+    assert (di, ii) == (st.div_iter, st.incom_iter)
+    compare_fields(sim, o, FIELDS_F, label="ref-surface")
+    compare_fields(sim, o, ["sph_density", "alpha", "W"], obj=1, label="ref-surface")
+
+
+@pytest.mark.parametrize("name", ["dambreak_2d_8k", "dambreak_3d_8k"])
+def test_fused_step_matches_oracle(name):
+    sim, o = make(name)
+    s = sim.sph_step(stats=True)
+    st = o.step(O.STEP_CFL)
+    print(f"  iters gpu=({s.div_iter},{s.incom_iter}) oracle=({st.div_iter},{st.incom_iter})", flush=True)
+    assert (s.div_iter, s.incom_iter) == (st.div_iter, st.incom_iter)
+    assert s.overflow_f == 0 and s.overflow_b == 0
+    assert s.dt == st.dt
+    compare_fields(sim, o, FIELDS_F + ["alpha_1", "alpha_2", "normal"], label="fused")
+    compare_fields(sim, o, ["sph_density", "alpha", "W"], obj=1, label="fused")
+    for a, b in zip(s.div_err, list(st.div_err)[: st.div_iter]):
+        assert a == pytest.approx(b, rel=1e-5)
+    # error of GPU and FP32 oracle against the FP64 oracle
+    cb, sb = load_scene(name)
+    o64 = O.Oracle(cb, sb, 8)
+    o64.init_scenario()
+    o64.step(O.STEP_CFL)
+    for fld in ("pos", "vel", "sph_density", "alpha", "psi_adv"):
+        print(f"  vs FP64 {fld:12s} gpu={rel_err(sim.fluid.to_numpy(fld), o64.get(0, fld)):.2e} "
+              f"oracle32={rel_err(o.get(0, fld), o64.get(0, fld)):.2e}", flush=True)
+
+
+def test_fused_multi_step_statistics():
+    sim, o = make("dambreak_2d_8k")
+    for step in range(20):
+        s = sim.sph_step(stats=True)
+        st = o.step(O.STEP_CFL)
+        assert (s.div_iter, s.incom_iter) == (st.div_iter, st.incom_iter), step
+        for a, b in zip(s.incom_err, list(st.incom_err)[: st.incom_iter]):
+            assert a == pytest.approx(b, rel=1e-3), step
+    com_g = sim.fluid.to_numpy("pos").mean(0)
+    com_o = o.get(0, "pos").mean(0)
+    assert np.abs(com_g - com_o).max() < 1e-6
+    assert s.kinetic_energy == pytest.approx(o.scalar("ke"), rel=1e-4)
+
+
+def test_fp64_fused_step_matches_fp64_oracle():
+    sim, o = make("dambreak_3d_8k", precision=8)
+    s = sim.sph_step(stats=True)
+    st = o.step(O.STEP_CFL)
+    assert (s.div_iter, s.incom_iter) == (st.div_iter, st.incom_iter)
+    compare_fields(sim, o, ["pos", "vel", "sph_density", "alpha", "psi_adv"], tol=1e-10, label="fp64")
+
+
+def test_two_phase_fbm_surface_tension_step():
+    sim, o = make("two_phase_3d_4k", fbm=True)
+    rng = np.random.default_rng(0)
+    n = sim.fluid.part_num
+    vf = o.get(0, "volume_frac").astype(np.float64) + rng.uniform(-0.02, 0.02, (n, 2))
+    vf = np.clip(vf, 0, None)
+    vf = (vf / vf.sum(1, keepdims=True)).astype(np.float32)
+    o.set(0, "volume_frac", vf)
+    sim.fluid.from_numpy("volume_frac", vf)
+    sim.fluid.update_mass_all()
+    rd = o.get(0, "rest_density")
+    assert np.array_equal(sim.fluid.to_numpy("rest_density"), rd)
+    s = sim.sph_step(stats=True)
+    st = o.step(O.STEP_CFL | O.STEP_FBM)
+    assert (s.div_iter, s.incom_iter) == (st.div_iter, st.incom_iter)
+    assert s.fbm_rounds == st.fbm_rounds
+    compare_fields(sim, o, ["pos", "vel", "sph_compression", "alpha", "normal", "volume_frac",
+                            "rest_density", "mass"], label="two-phase")
+
+
+def test_full_size_sort_and_lists_properties():
+    """1M-particle scene: integer invariants that do not need the oracle."""
+    from tichi_sph_amd import load_simulation
+    from conftest import SCEN
+    import os
+    sim = load_simulation(os.path.join(SCEN, "dambreak_3d_1m.config.json"),
+                          os.path.join(SCEN, "dambreak_3d_1m.json"))
+    assert sim.fluid.part_num == 1_000_000 and sim.bound.part_num == 206_464
+    s = sim.sph_step(stats=True)
+    f = sim.fluid
+    n = f.part_num
+    key = f.t["cell"][:n].cpu().numpy()
+    pid = f.t["pid"][:n].cpu().numpy()
+    assert np.all(np.diff(key) >= 0)
+    assert np.all(np.diff(pid)[np.diff(key) == 0] > 0)
+    assert np.array_equal(np.sort(pid), np.arange(n))
+    assert s.overflow_f == 0 and s.overflow_b == 0
+    nf = f.t["nbr_nf"][:n].cpu().numpy()
+    assert nf.max() <= 64 and nf.min() >= 1
+    assert 2 <= s.div_iter <= 101 and 2 <= s.incom_iter <= 101
+    # momentum balance of a step from rest: total fluid momentum change equals
+    # gravity impulse plus wall forces (walls push up: net y-momentum above -M g dt)
+    m = f.to_numpy("mass").astype(np.float64)
+    v = f.to_numpy("vel").astype(np.float64)
+    assert np.isfinite(v).all()

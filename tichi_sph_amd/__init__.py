@@ -1,0 +1,11 @@
+"""MI355X (gfx950) implementation of Tichi_SPH's per-step particle hot path.
+
+Hand-written HIP kernels (tichi_sph_amd/csrc) behind the C ABI of
+include/tichi_sph.h, driven from Python through ctypes on torch device
+tensors.  The Python surface keeps the reference's names: Config,
+Fluid, Ngrid, NeighborSearch, the sph.py kernel functions and sph_step().
+"""
+from .config import Config, read_param  # noqa: F401
+from .sph_obj import Fluid, Ngrid  # noqa: F401
+from .solver import Simulation, StepStats, load_simulation, simulation_from_dicts  # noqa: F401
+from .NeighborSearch import NeighborSearch  # noqa: F401

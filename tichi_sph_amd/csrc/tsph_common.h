@@ -1,0 +1,279 @@
+// Shared device-side definitions of the MI355X SPH hot path.
+//
+// Two flavours of pair arithmetic live here:
+//  * strict: the reference's evaluation order (sph_util.py:4-43, sph.py) with
+//    every operation rounded once (no contraction, correctly rounded divide
+//    and sqrt).  Used where last-ulp agreement matters: cell keys, the
+//    in-range test, the density sums (which the solvers difference against
+//    the rest density) and the reference-surface kernels.
+//  * fast: algebraically equal forms with one reciprocal square root per
+//    pair.  Used by the Jacobi and advection sweeps.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/tichi_sph.h"
+
+namespace tsph {
+
+constexpr int kBlock = 256;     // threads per block (4 wave64)
+constexpr int kWaves = kBlock / 64;
+constexpr int kMaxP = TSPH_MAX_PHASE;
+
+template <class T>
+struct alignas(4 * sizeof(T)) V4 {
+  T x, y, z, w;
+};
+
+// Typed per-scene constants, built on the host from TsphParams.  Values the
+// reference keeps in f32 fields are exact in T.
+template <class T>
+struct KP {
+  int dim, P, solver, ntemp, node_num, R;
+  int gs[3], coder[3];
+  int list_cap_f, list_cap_b;
+  int iter_min, iter_max, cs, wc_gamma;
+  T lb[3], g[3], rho[kMaxP];
+  T h, h2, sig, sig_h, gps, c_coh, eps, r2max, inv_h;
+  T mu, gamma, fbmD, fbmC, cfl;
+  T div_thr, comp_thr;
+  T two_d2;      // 2*(2+dim) of W_lap (sph_util.py:42)
+  T wg_a, wg_b;  // fast gradient factors: 6*sig/h^2 and 6*sig/h
+};
+
+// ---------------------------------------------------------------------------
+// Block mapping.  Workgroups are dispatched round-robin over the 8 XCDs
+// (block b runs on XCD b % 8); remapping gives each XCD one contiguous range
+// of particles, so a sweep's neighbour halo is shared inside one L2.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  int q = nb >> 3, rem = nb & 7, x = b & 7, idx = b >> 3;
+  return x * q + (x < rem ? x : rem) + idx;
+}
+
+// ---------------------------------------------------------------------------
+// Strict arithmetic (reference evaluation order, single rounding per op).
+// ---------------------------------------------------------------------------
+#define TSPH_STRICT _Pragma("clang fp contract(off)")
+
+template <class T>
+__device__ __forceinline__ T s_sqrt(T x) {
+  return __builtin_sqrt(x);
+}
+template <>
+__device__ __forceinline__ float s_sqrt<float>(float x) {
+  return __builtin_sqrtf(x);
+}
+
+// ((x0*x0) + (x1*x1)) + (x2*x2), the reference's norm_sqr (z = 0 in 2D adds +0)
+template <class T>
+__device__ __forceinline__ T s_r2(T dx, T dy, T dz) {
+  TSPH_STRICT
+  T a = dx * dx;
+  T b = dy * dy;
+  T c = dz * dz;
+  T s = a + b;
+  return s + c;
+}
+
+template <class T>
+__device__ __forceinline__ T s_pow3(T x) {
+  TSPH_STRICT
+  T t = x * x;
+  return x * t;
+}
+
+// W, sph_util.py:16-25
+template <class T>
+__device__ __forceinline__ T s_W(const KP<T>& k, T r) {
+  TSPH_STRICT
+  T q = r / k.h;
+  T tmp = T(0);
+  if (q <= T(0.5)) {
+    T q3 = s_pow3(q);
+    T q2 = q * q;
+    tmp = T(6) * (q3 - q2) + T(1);
+  } else if (q > T(0.5) && q < T(1)) {
+    tmp = T(2) * s_pow3(T(1) - q);
+  }
+  return tmp * k.sig;
+}
+
+// W_grad, sph_util.py:28-37
+template <class T>
+__device__ __forceinline__ T s_Wg(const KP<T>& k, T r) {
+  TSPH_STRICT
+  T q = r / k.h;
+  T tmp = T(0);
+  if (q <= T(0.5)) {
+    T q2 = q * q;
+    T a = T(3) * q2;
+    T b = T(2) * q;
+    tmp = T(6) * (a - b);
+  } else if (q > T(0.5) && q < T(1)) {
+    T u = T(1) - q;
+    tmp = T(-6) * (u * u);
+  }
+  return tmp * k.sig_h;
+}
+
+// C, sph_util.py:4-13
+template <class T>
+__device__ __forceinline__ T s_C(const KP<T>& k, T r) {
+  TSPH_STRICT
+  T q = r / k.h;
+  T tmp = T(0);
+  if (q <= T(0.5)) {
+    T a = T(2) * s_pow3(T(1) - q);
+    tmp = a * s_pow3(q) - T(1.0 / 64);
+  } else if (q > T(0.5) && q < T(1)) {
+    tmp = s_pow3(T(1) - q) * s_pow3(q);
+  }
+  return tmp * k.c_coh;
+}
+
+// ---------------------------------------------------------------------------
+// Fast arithmetic.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ T f_rsqrt(T x) {
+  return rsqrt(x);
+}
+template <>
+__device__ __forceinline__ float f_rsqrt<float>(float x) {
+  return __builtin_amdgcn_rsqf(x);
+}
+template <class T>
+__device__ __forceinline__ T f_rcp(T x) {
+  return T(1) / x;
+}
+template <>
+__device__ __forceinline__ float f_rcp<float>(float x) {
+  return __builtin_amdgcn_rcpf(x);
+}
+
+// g = W_grad(r)/r so that grad W_ij = g * x_ij, for 0 < r2 < r2max.
+// Returns r through *r_out.
+template <class T>
+__device__ __forceinline__ T f_grad(const KP<T>& k, T r2, T* r_out) {
+  T rinv = f_rsqrt(r2);
+  T r = r2 * rinv;
+  T q = r * k.inv_h;
+  T u = T(1) - q;
+  // q <= 1/2: 6 sig/h (3q^2 - 2q) / r = 6 sig/h^2 (3q - 2)
+  // q >  1/2: -6 sig/h (1 - q)^2 / r
+  T ga = k.wg_a * (T(3) * q - T(2));
+  T gb = -k.wg_b * (u * u) * rinv;
+  *r_out = r;
+  return q <= T(0.5) ? ga : gb;
+}
+
+// ---------------------------------------------------------------------------
+// Cell coding, sph_util.py:65-72 (floor division in IEEE, int32 wrapping).
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ int to_i32(T v) {
+  if (!(v >= T(-2147483648.0) && v < T(2147483648.0))) return INT32_MIN;
+  return (int)v;
+}
+
+template <class T>
+__device__ __forceinline__ void node_encode(const KP<T>& k, T x, T y, T z, int* c) {
+  TSPH_STRICT
+  c[0] = to_i32(floor((x - k.lb[0]) / k.h));
+  c[1] = to_i32(floor((y - k.lb[1]) / k.h));
+  c[2] = k.dim == 3 ? to_i32(floor((z - k.lb[2]) / k.h)) : 0;
+}
+
+__device__ __forceinline__ int wadd(int a, int b) { return (int)((uint32_t)a + (uint32_t)b); }
+__device__ __forceinline__ int wmul(int a, int b) { return (int)((uint32_t)a * (uint32_t)b); }
+
+template <class T>
+__device__ __forceinline__ int dim_encode(const KP<T>& k, int cx, int cy, int cz) {
+  int s = wmul(k.coder[0], cx);
+  s = wadd(s, wmul(k.coder[1], cy));
+  if (k.dim == 3) s = wadd(s, wmul(k.coder[2], cz));
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// Reductions (wave64 shuffles, then LDS across the block's waves).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_or(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Sum over the block; the result is valid in thread 0.  Fixed order for a
+// fixed block size, so the value is deterministic.
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+  v = wave_sum(v);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) lds[w] = v;
+  __syncthreads();
+  double s = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += lds[i];
+  __syncthreads();
+  return s;
+}
+__device__ __forceinline__ double block_min(double v, double* lds) {
+  v = wave_min(v);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) lds[w] = v;
+  __syncthreads();
+  double s = v;
+  if (threadIdx.x == 0) {
+    s = lds[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) s = fmin(s, lds[i]);
+  }
+  __syncthreads();
+  return s;
+}
+
+// Last-block election: every block calls this once after writing its
+// partial; returns true in exactly one block (the last to arrive), after
+// which that block sees every other block's partial.  The counter is reset
+// for the next launch by the elected block.
+__device__ __forceinline__ bool elect_last_block(int* counter, int nblocks, int* lds_flag) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    int prev = atomicAdd(counter, 1);
+    *lds_flag = (prev == nblocks - 1);
+  }
+  __syncthreads();
+  bool last = *lds_flag != 0;
+  if (last) {
+    __threadfence();
+    if (threadIdx.x == 0) *counter = 0;
+  }
+  return last;
+}
+
+// Deterministic fixed-order sum of `n` partials by one block (result in
+// thread 0).
+__device__ __forceinline__ double sum_partials(const double* part, int n, double* lds) {
+  double v = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) v += __builtin_nontemporal_load(part + i);
+  return block_sum(v, lds);
+}
+__device__ __forceinline__ double min_partials(const double* part, int n, double* lds, double init) {
+  double v = init;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) v = fmin(v, __builtin_nontemporal_load(part + i));
+  return block_min(v, lds);
+}
+
+}  // namespace tsph

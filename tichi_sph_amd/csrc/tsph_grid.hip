@@ -1,0 +1,356 @@
+// Neighbour grid kernels.
+//
+// Fused path (tsph_sort_object): a counting sort that moves one particle
+// object into cell order every step.
+//   keys      cell key per particle (node_encode + dim_encode,
+//             sph_util.py:65-72), arrival rank via an atomic per cell
+//             (encode, sph_obj.py:300-306); optional cfl_condition partials
+//   scan      exclusive prefix sum of the per-cell counts (mem_shift,
+//             sph_obj.py:308-314, in its single-thread meaning)
+//   scatter   provisional slot = start[key] + arrival rank
+//   permute   final slot = start[key] + rank of the particle's reference
+//             index among its cell-mates (fill_node's single-thread order,
+//             sph_obj.py:316-323), then the persistent state is gathered
+//             into the back buffers.
+// Reference layout (tsph_ngrid_build): the reference's shared Ngrid arrays
+// over unsorted storage, same algorithm over (cell, uid) buckets.
+#include "tsph_host.h"
+
+namespace tsph {
+
+// ---------------------------------------------------------------------------
+// Scan: exclusive prefix of `cnt[0..len)` into `start[0..len]`.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_scan_reduce(const int* __restrict__ cnt, long len,
+                                                        int* __restrict__ sums) {
+  __shared__ double lds[kWaves];
+  long base = (long)blockIdx.x * kScanTile;
+  long s = 0;
+  for (int e = 0; e < 16; ++e) {
+    long i = base + (long)e * kBlock + threadIdx.x;
+    if (i < len) s += cnt[i];
+  }
+  double t = block_sum((double)s, lds);
+  if (threadIdx.x == 0) sums[blockIdx.x] = (int)t;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_apply(const int* __restrict__ cnt, long len,
+                                                       const int* __restrict__ sums,
+                                                       int* __restrict__ start) {
+  __shared__ int wsum[kWaves];
+  __shared__ int boff;
+  // offset of this tile = sum of previous tile sums
+  if (threadIdx.x < 64) {
+    int acc = 0;
+    for (int i = threadIdx.x; i < (int)blockIdx.x; i += 64) acc += sums[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (threadIdx.x == 0) boff = acc;
+  }
+  long base = (long)blockIdx.x * kScanTile + (long)threadIdx.x * 16;
+  int v[16];
+  int loc = 0;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    long i = base + e;
+    v[e] = i < len ? cnt[i] : 0;
+    loc += v[e];
+  }
+  // inclusive scan of per-thread totals across the block
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = loc;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  int wo = 0;
+  for (int i = 0; i < w; ++i) wo += wsum[i];
+  int run = boff + wo + x - loc;  // exclusive prefix of this thread's first element
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    long i = base + e;
+    if (i < len) start[i] = run;
+    run += v[e];
+  }
+  // start[len] = total, written by the thread owning the last element
+  if (base <= len - 1 && len - 1 < base + 16) start[len] = run;
+}
+
+static int launch_scan(const int* cnt, long len, int* sums, int* start, hipStream_t st) {
+  long nb = scan_blocks(len);
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, st, cnt, len, sums);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kBlock), 0, st, cnt, len, sums, start);
+  TSPH_LAUNCH_CHECK("scan");
+  return 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill_i32(int* __restrict__ p, long n, int v) {
+  long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Fused-path sort.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_keys(KP<T> k, const V4<T>* __restrict__ pos, const V4<T>* __restrict__ vel, int n,
+           int* __restrict__ key, int* __restrict__ rank, int* __restrict__ count, int cfl,
+           double* __restrict__ partial, TsphControl* ctrl) {
+  __shared__ double lds[kWaves];
+  __shared__ int flag;
+  int nb = gridDim.x;
+  int b = xcd_block(blockIdx.x, nb);
+  int i = b * kBlock + threadIdx.x;
+  double cand = 1e300;
+  if (i < n) {
+    V4<T> p = pos[i];
+    int c[3];
+    node_encode(k, p.x, p.y, p.z, c);
+    int code = dim_encode(k, c[0], c[1], c[2]);
+    int kk = (0 < code && code < k.node_num) ? code : k.node_num;  // invalid -> trailing bucket
+    key[i] = kk;
+    rank[i] = atomicAdd(&count[kk], 1);
+    if (cfl) {  // cfl_condition, sph.py:44-47
+      TSPH_STRICT
+      V4<T> v = vel[i];
+      T vn = s_sqrt(s_r2(v.x, v.y, v.z));
+      if (vn > T(1e-4)) {
+        T q = k.gps / vn;
+        cand = (double)(q * k.cfl);
+      }
+    }
+  }
+  if (!cfl) return;
+  double m = block_min(cand, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = m;
+  if (elect_last_block(&ctrl->counter, nb, &flag)) {
+    double all = min_partials(partial, nb, lds, 1e300);
+    if (threadIdx.x == 0) {
+      TSPH_STRICT
+      T dt0 = k.gps / (T)k.cs;  // sph.py:43
+      T d = dt0;
+      if (all < (double)dt0) d = (T)all;
+      ctrl->dt = (double)d;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_scatter(const int* __restrict__ key, const int* __restrict__ rank,
+              const int* __restrict__ start, const int* __restrict__ pid, int n,
+              int* __restrict__ tmp_idx, int* __restrict__ tmp_pid, int* __restrict__ tmp_key) {
+  int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  int kk = key[i];
+  int s = start[kk] + rank[i];
+  tmp_idx[s] = i;
+  tmp_pid[s] = pid[i];
+  tmp_key[s] = kk;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_permute(int P, int n, const int* __restrict__ start, const int* __restrict__ tmp_idx,
+              const int* __restrict__ tmp_pid, const int* __restrict__ tmp_key,
+              const V4<T>* __restrict__ pos, const V4<T>* __restrict__ vel,
+              const T* __restrict__ mass, const T* __restrict__ rdens, const T* __restrict__ rvol,
+              const T* __restrict__ vf, const int* __restrict__ pid, V4<T>* __restrict__ pos_b,
+              V4<T>* __restrict__ vel_b, T* __restrict__ mass_b, T* __restrict__ rdens_b,
+              T* __restrict__ rvol_b, T* __restrict__ vf_b, int* __restrict__ pid_b,
+              int* __restrict__ cell) {
+  int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (s >= n) return;
+  int c = tmp_key[s];
+  int a = start[c], e = start[c + 1];
+  int p = tmp_pid[s];
+  int r = 0;
+  for (int t = a; t < e; ++t) r += tmp_pid[t] < p;
+  int dst = a + r;
+  int src = tmp_idx[s];
+  pos_b[dst] = pos[src];
+  vel_b[dst] = vel[src];
+  mass_b[dst] = mass[src];
+  rdens_b[dst] = rdens[src];
+  rvol_b[dst] = rvol[src];
+  for (int q = 0; q < P; ++q) vf_b[(long)dst * P + q] = vf[(long)src * P + q];
+  pid_b[dst] = p;
+  cell[dst] = c;
+}
+
+template <class T>
+int sort_object(const TsphParams& prm, TsphObject* o, TsphControl* ctrl, int cfl, void* ws,
+                size_t wsb, hipStream_t st) {
+  KP<T> k = make_kp<T>(prm);
+  int n = o->n;
+  WsLayout L;
+  TSPH_REQUIRE(carve_ws(ws, wsb, o->capacity, prm.node_num, &L), TSPH_ECAPACITY,
+               "tsph_sort_object: workspace too small (%zu bytes)", wsb);
+  long len = (long)prm.node_num + 1;  // cells + trailing invalid bucket
+  hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(len + 1)), dim3(kBlock), 0, st, o->cell_count,
+                     len + 1, 0);
+  int nb = blocks_for(n);
+  if (n > 0 || cfl) {
+    int nbl = nb > 0 ? nb : 1;
+    hipLaunchKernelGGL(k_keys<T>, dim3(nbl), dim3(kBlock), 0, st, k, (const V4<T>*)o->pos,
+                       (const V4<T>*)o->vel, n, L.key, L.rank, o->cell_count, cfl, L.partial,
+                       ctrl);
+  }
+  TSPH_LAUNCH_CHECK("keys");
+  if (int rc = launch_scan(o->cell_count, len, L.scan_sums, o->cell_start, st)) return rc;
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kBlock), 0, st, L.key, L.rank, o->cell_start,
+                     o->pid, n, L.tmp_idx, L.tmp_pid, L.tmp_key);
+  hipLaunchKernelGGL(k_permute<T>, dim3(nb), dim3(kBlock), 0, st, prm.phase_num, n,
+                     o->cell_start, L.tmp_idx, L.tmp_pid, L.tmp_key, (const V4<T>*)o->pos,
+                     (const V4<T>*)o->vel, (const T*)o->mass, (const T*)o->rest_density,
+                     (const T*)o->rest_volume, (const T*)o->volume_frac, o->pid,
+                     (V4<T>*)o->pos_b, (V4<T>*)o->vel_b, (T*)o->mass_b, (T*)o->rest_density_b,
+                     (T*)o->rest_volume_b, (T*)o->volume_frac_b, o->pid_b, o->cell);
+  TSPH_LAUNCH_CHECK("permute");
+  std::swap(o->pos, o->pos_b);
+  std::swap(o->vel, o->vel_b);
+  std::swap(o->mass, o->mass_b);
+  std::swap(o->rest_density, o->rest_density_b);
+  std::swap(o->rest_volume, o->rest_volume_b);
+  std::swap(o->volume_frac, o->volume_frac_b);
+  std::swap(o->pid, o->pid_b);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Reference-layout Ngrid (shared by all objects, sph_obj.py:280-323).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_ref_encode(KP<T> k, const V4<T>* __restrict__ pos, int n, int u, int nobj,
+                 int* __restrict__ cell, int* __restrict__ cell_structured,
+                 int* __restrict__ rank, int* __restrict__ cu_count) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  V4<T> p = pos[i];
+  int c[3];
+  node_encode(k, p.x, p.y, p.z, c);
+  int code = dim_encode(k, c[0], c[1], c[2]);
+  cell[i] = code;
+  cell_structured[4 * i + 0] = c[0];
+  cell_structured[4 * i + 1] = c[1];
+  cell_structured[4 * i + 2] = c[2];
+  cell_structured[4 * i + 3] = 0;
+  if (0 < code && code < k.node_num) rank[i] = atomicAdd(&cu_count[(long)code * nobj + u], 1);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_ref_cells(int node_num, int nobj, const int* __restrict__ cu_count,
+                const int* __restrict__ cu_start, int* __restrict__ npc, int* __restrict__ nps,
+                int* __restrict__ npsc) {
+  int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= node_num) return;
+  int s = 0;
+  for (int u = 0; u < nobj; ++u) s += cu_count[(long)c * nobj + u];
+  npc[c] = s;
+  nps[c] = cu_start[(long)c * nobj];
+  npsc[c] = cu_start[(long)c * nobj] + s;  // shift_count after fill_node
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_ref_scatter(int node_num, const int* __restrict__ cell, const int* __restrict__ rank,
+                  const int* __restrict__ pid, int n, int u, int nobj,
+                  const int* __restrict__ cu_start, int* __restrict__ tmp_idx,
+                  int* __restrict__ tmp_pid, int* __restrict__ tmp_key) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  int code = cell[i];
+  if (!(0 < code && code < node_num)) return;
+  long b = (long)code * nobj + u;
+  int s = cu_start[b] + rank[i];
+  tmp_idx[s] = i;
+  tmp_pid[s] = pid[i];
+  tmp_key[s] = (int)b;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_ref_fill(int cap, const int* __restrict__ total_ptr, const int* __restrict__ cu_start, const int* __restrict__ tmp_idx,
+               const int* __restrict__ tmp_pid, const int* __restrict__ tmp_key, int nobj,
+               int* __restrict__ pid_in_node, int* __restrict__ uid_in_node, int* in_cell0,
+               int* in_cell1, int* in_cell2, int* in_cell3) {
+  int s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= cap || s >= *total_ptr) return;
+  int b = tmp_key[s];
+  int a = cu_start[b], e = cu_start[b + 1];
+  int p = tmp_pid[s];
+  int r = 0;
+  for (int t = a; t < e; ++t) r += tmp_pid[t] < p;
+  int dst = a + r;
+  int u = b % nobj;
+  int i = tmp_idx[s];
+  pid_in_node[dst] = i;
+  uid_in_node[dst] = u;
+  int* ic = u == 0 ? in_cell0 : u == 1 ? in_cell1 : u == 2 ? in_cell2 : in_cell3;
+  ic[i] = dst;
+}
+
+template <class T>
+int ngrid_build(const TsphParams& prm, TsphObject* const* objs, int nobj, int* npc, int* nps,
+                int* npsc, int* pin, int* uin, void* ws, size_t wsb, hipStream_t st) {
+  TSPH_REQUIRE(nobj >= 1 && nobj <= 2, TSPH_EINVAL, "tsph_ngrid_build: 1 or 2 objects");
+  KP<T> k = make_kp<T>(prm);
+  long cap = 0, total_cap = 0;
+  for (int u = 0; u < nobj; ++u) {
+    TSPH_REQUIRE(objs[u]->uid == u, TSPH_EINVAL, "tsph_ngrid_build: object %d has uid %d", u,
+                 objs[u]->uid);
+    cap = std::max<long>(cap, objs[u]->capacity);
+    total_cap += objs[u]->capacity;
+  }
+  WsLayout L;
+  TSPH_REQUIRE(carve_ws(ws, wsb, total_cap, prm.node_num, &L), TSPH_ECAPACITY,
+               "tsph_ngrid_build: workspace too small (%zu bytes)", wsb);
+  long lenref = (long)prm.node_num * nobj;
+  hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(lenref + 1)), dim3(kBlock), 0, st, L.cu_count,
+                     lenref + 1, 0);
+  // encode (sph_obj.py:300-306); rank temporaries are per object
+  long roff = 0;
+  for (int u = 0; u < nobj; ++u) {
+    TsphObject* o = objs[u];
+    if (o->n > 0)
+      hipLaunchKernelGGL(k_ref_encode<T>, dim3(blocks_for(o->n)), dim3(kBlock), 0, st, k,
+                         (const V4<T>*)o->pos, o->n, u, nobj, o->cell, o->cell_structured,
+                         L.rank + roff, L.cu_count);
+    roff += o->capacity;
+  }
+  TSPH_LAUNCH_CHECK("ngrid encode");
+  // mem_shift (sph_obj.py:308-314)
+  if (int rc = launch_scan(L.cu_count, lenref, L.scan_sums, L.cu_start, st)) return rc;
+  hipLaunchKernelGGL(k_ref_cells, dim3(blocks_for(prm.node_num)), dim3(kBlock), 0, st,
+                     prm.node_num, nobj, L.cu_count, L.cu_start, npc, nps, npsc);
+  // fill_node (sph_obj.py:316-323)
+  roff = 0;
+  for (int u = 0; u < nobj; ++u) {
+    TsphObject* o = objs[u];
+    if (o->n > 0)
+      hipLaunchKernelGGL(k_ref_scatter, dim3(blocks_for(o->n)), dim3(kBlock), 0, st,
+                         prm.node_num, o->cell, L.rank + roff, o->pid, o->n, u, nobj, L.cu_start,
+                         L.tmp_idx, L.tmp_pid, L.tmp_key);
+    roff += o->capacity;
+  }
+  // total entries = cu_start[lenref]; launch over total capacity, extra threads exit
+  long tc = total_cap;
+  int* ic[4] = {nullptr, nullptr, nullptr, nullptr};
+  for (int u = 0; u < nobj; ++u) ic[u] = objs[u]->in_cell;
+  hipLaunchKernelGGL(k_ref_fill, dim3(blocks_for(tc)), dim3(kBlock), 0, st, (int)tc,
+                     L.cu_start + lenref, L.cu_start, L.tmp_idx, L.tmp_pid, L.tmp_key, nobj, pin,
+                     uin, ic[0], ic[1], ic[2], ic[3]);
+  TSPH_LAUNCH_CHECK("ngrid fill");
+  return 0;
+}
+
+template int sort_object<float>(const TsphParams&, TsphObject*, TsphControl*, int, void*, size_t, hipStream_t);
+template int sort_object<double>(const TsphParams&, TsphObject*, TsphControl*, int, void*, size_t, hipStream_t);
+template int ngrid_build<float>(const TsphParams&, TsphObject* const*, int, int*, int*, int*, int*, int*, void*, size_t, hipStream_t);
+template int ngrid_build<double>(const TsphParams&, TsphObject* const*, int, int*, int*, int*, int*, int*, void*, size_t, hipStream_t);
+
+}  // namespace tsph

@@ -1,0 +1,780 @@
+// Fused per-step kernels of the DFSPH / VFSPH step (main.py:175-256).
+//
+// Storage is in cell order (tsph_grid.hip).  The 27 (3D) / 9 (2D) cells of a
+// particle's neighbourhood are visited as 9 / 3 contiguous row runs: cells
+// (x-R .. x+R) of one (y, z) row have consecutive codes and therefore
+// consecutive slots, in the reference's template order (x fastest,
+// sph_config.py:219-224) and in-cell order.  Pass A walks the runs once per
+// step, testing every candidate, and records the in-range neighbours
+// (0 < r < h, exact f32 test) of each fluid particle in reference order;
+// every later sweep iterates only those lists.  Terms beyond h are exact
+// zeros in the reference, so skipping them changes no value.
+#include "tsph_host.h"
+
+namespace tsph {
+
+// Visit every candidate slot of `cstart`'s object in the cells around c[].
+template <class T, class F>
+__device__ __forceinline__ void walk_cells(const KP<T>& k, const int* __restrict__ cstart,
+                                           const int* c, F&& f) {
+  const int R = k.R;
+  const int zr = k.dim == 3 ? R : 0;
+#pragma unroll 1
+  for (int dz = -zr; dz <= zr; ++dz) {
+#pragma unroll 1
+    for (int dy = -R; dy <= R; ++dy) {
+      int c0 = dim_encode(k, wadd(c[0], -R), wadd(c[1], dy), wadd(c[2], dz));
+      long lo = c0 < 1 ? 1 : (long)c0;
+      long hi = (long)c0 + 2 * R;
+      if (hi > k.node_num - 1) hi = k.node_num - 1;
+      if (lo > hi) continue;
+      int j0 = cstart[lo], j1 = cstart[hi + 1];
+      for (int j = j0; j < j1; ++j) f(j);
+    }
+  }
+}
+
+template <class T>
+__device__ __forceinline__ T rest_psi_of(const KP<T>& k, const T* rdens, int i) {
+  return k.solver == TSPH_VFSPH ? T(1) : rdens[i];
+}
+
+// ---------------------------------------------------------------------------
+// Pass A, fluid particles: SPH_prepare_attr (f,f)(f,b), SPH_prepare_alpha_1
+// (f,f)(f,b), SPH_prepare_alpha_2 (f,f), SPH_prepare_alpha, SPH_vel_2_vel_adv,
+// neighbour lists.  Strict arithmetic: bit-for-bit the reference's order.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_pass_a_fluid(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
+                   const T* __restrict__ fmass, const T* __restrict__ frvol,
+                   const int* __restrict__ fstart, int nb, const V4<T>* __restrict__ bpos,
+                   const T* __restrict__ bmass, const T* __restrict__ brvol,
+                   const int* __restrict__ bstart, T* __restrict__ W, T* __restrict__ comp,
+                   T* __restrict__ dens, V4<T>* __restrict__ alpha_1, T* __restrict__ alpha_2,
+                   T* __restrict__ alpha, T* __restrict__ vol, const V4<T>* __restrict__ vel,
+                   V4<T>* __restrict__ vel_adv, int* __restrict__ nbr_f, int* __restrict__ nbr_b,
+                   int* __restrict__ nbr_nf, int* __restrict__ nbr_nb, int* __restrict__ flag,
+                   TsphControl* ctrl) {
+  TSPH_STRICT
+  int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  flag[i] = 0;  // SPH_clean_value, sph.py:30
+  V4<T> p = fpos[i];
+  T mi = fmass[i];
+  int c[3];
+  node_encode(k, p.x, p.y, p.z, c);
+  T w = 0, cp = 0, dn = 0, ax = 0, ay = 0, az = 0, a2 = 0;
+  int nf = 0, nbb = 0;
+  const int capf = k.list_cap_f, capb = k.list_cap_b;
+  walk_cells(k, fstart, c, [&](int j) {
+    V4<T> q = fpos[j];
+    T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    T r2 = s_r2(dx, dy, dz);
+    if (r2 <= k.r2max) {
+      T r = s_sqrt(r2);
+      T Wr = s_W(k, r);
+      w = w + Wr;
+      cp = cp + Wr * frvol[j];
+      dn = dn + Wr * fmass[j];
+      if (r > T(0)) {
+        T wg = s_Wg(k, r);
+        T Xj = q.w;
+        ax = ax + Xj * dx / r * wg;
+        ay = ay + Xj * dy / r * wg;
+        az = az + Xj * dz / r * wg;
+        T wg2 = wg * wg;
+        a2 = a2 + wg2 * (Xj * Xj) / fmass[j];
+        if (nf < capf) nbr_f[(long)nf * cap + i] = j;
+        ++nf;
+      }
+    }
+  });
+  walk_cells(k, bstart, c, [&](int j) {
+    V4<T> q = bpos[j];
+    T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    T r2 = s_r2(dx, dy, dz);
+    if (r2 <= k.r2max) {
+      T r = s_sqrt(r2);
+      T Wr = s_W(k, r);
+      w = w + Wr;
+      cp = cp + Wr * brvol[j];
+      dn = dn + Wr * bmass[j];
+      if (r > T(0)) {
+        T wg = s_Wg(k, r);
+        T Xj = q.w;
+        ax = ax + Xj * dx / r * wg;
+        ay = ay + Xj * dy / r * wg;
+        az = az + Xj * dz / r * wg;
+        if (nbb < capb) nbr_b[(long)nbb * cap + i] = j;
+        ++nbb;
+      }
+    }
+  });
+  W[i] = w;
+  comp[i] = cp;
+  dens[i] = dn;
+  alpha_1[i] = V4<T>{ax, ay, az, T(0)};
+  alpha_2[i] = a2;
+  T dot = s_r2(ax, ay, az);
+  T al = dot / mi + a2;  // SPH_prepare_alpha, sph.py:103-105
+  if (al < T(1e-4)) al = T(1e-4);
+  alpha[i] = al;
+  T psi = k.solver == TSPH_VFSPH ? cp : dn;
+  vol[i] = p.w / psi;  // X / sph_psi, reused by the viscosity and normal sweeps
+  vel_adv[i] = vel[i];  // SPH_vel_2_vel_adv, sph.py:253-256
+  if (nf > capf) {
+    atomicAdd(&ctrl->overflow_f, 1);
+    nf = capf + 1;
+  }
+  if (nbb > capb) {
+    atomicAdd(&ctrl->overflow_b, 1);
+    nbb = capb + 1;
+  }
+  nbr_nf[i] = nf;
+  nbr_nb[i] = nbb;
+}
+
+// Pass A, boundary particles: SPH_prepare_attr (b,b)(b,f), SPH_prepare_alpha_2
+// (b,f), SPH_prepare_alpha (alpha_1 of the boundary is never accumulated).
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_pass_a_bound(KP<T> k, int nb, const V4<T>* __restrict__ bpos, const T* __restrict__ bmass,
+                   const T* __restrict__ brvol, const int* __restrict__ bstart,
+                   const V4<T>* __restrict__ fpos, const T* __restrict__ fmass,
+                   const T* __restrict__ frvol, const int* __restrict__ fstart,
+                   T* __restrict__ W, T* __restrict__ comp, T* __restrict__ dens,
+                   V4<T>* __restrict__ alpha_1, T* __restrict__ alpha_2, T* __restrict__ alpha,
+                   T* __restrict__ vol) {
+  TSPH_STRICT
+  int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= nb) return;
+  V4<T> p = bpos[i];
+  int c[3];
+  node_encode(k, p.x, p.y, p.z, c);
+  T w = 0, cp = 0, dn = 0, a2 = 0;
+  walk_cells(k, bstart, c, [&](int j) {
+    V4<T> q = bpos[j];
+    T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
+    if (r2 <= k.r2max) {
+      T Wr = s_W(k, s_sqrt(r2));
+      w = w + Wr;
+      cp = cp + Wr * brvol[j];
+      dn = dn + Wr * bmass[j];
+    }
+  });
+  walk_cells(k, fstart, c, [&](int j) {
+    V4<T> q = fpos[j];
+    T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
+    if (r2 <= k.r2max) {
+      T r = s_sqrt(r2);
+      T Wr = s_W(k, r);
+      w = w + Wr;
+      cp = cp + Wr * frvol[j];
+      dn = dn + Wr * fmass[j];
+      if (r > T(0)) {
+        T wg = s_Wg(k, r);
+        T Xj = q.w;
+        T wg2 = wg * wg;
+        a2 = a2 + wg2 * (Xj * Xj) / fmass[j];
+      }
+    }
+  });
+  W[i] = w;
+  comp[i] = cp;
+  dens[i] = dn;
+  alpha_1[i] = V4<T>{T(0), T(0), T(0), T(0)};
+  alpha_2[i] = a2;
+  T al = T(0) / bmass[i] + a2;
+  if (al < T(1e-4)) al = T(1e-4);
+  alpha[i] = al;
+  T psi = k.solver == TSPH_VFSPH ? cp : dn;
+  vol[i] = p.w / psi;
+}
+
+// Per-step reset of the control block's step-scoped state: general_flag = 1
+// (SPH_clean_value, sph.py:24), FBM round count, list-overflow counters.
+__global__ void k_step_reset(TsphControl* ctrl) {
+  if (threadIdx.x == 0) {
+    ctrl->general_flag = 1;
+    ctrl->fbm_rounds = 0;
+    ctrl->overflow_f = 0;
+    ctrl->overflow_b = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Solver stop state (main.py:201-213 / 222-234).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void k_solver_begin(KP<T> k, TsphControl* ctrl, int solve) {
+  int t = threadIdx.x;
+  for (int q = t; q < TSPH_MAX_ITER_LOG; q += blockDim.x) ctrl->run_u[q] = 0;
+  if (t == 0) {
+    ctrl->it = 0;
+    ctrl->solve = solve;
+    T thr = solve == 0 ? k.div_thr : k.comp_thr;
+    T comp = (T)ctrl->compression;
+    bool enter = (0 < k.iter_min) || (comp > thr);
+    ctrl->stop = enter ? 0 : 1;
+    if (solve == 0) ctrl->div_iter = 0;
+    else ctrl->incom_iter = 0;
+  }
+}
+
+// Iterate a particle's fluid neighbours: the recorded list, or, when the list
+// overflowed, the cell walk with the same in-range test.
+template <class T, class F>
+__device__ __forceinline__ void for_fluid_nbrs(const KP<T>& k, int i, int cap, int nf,
+                                               const int* __restrict__ list,
+                                               const int* __restrict__ cstart,
+                                               const V4<T>* __restrict__ pos, V4<T> p, F&& f) {
+  if (nf <= k.list_cap_f) {
+    for (int s = 0; s < nf; ++s) f(list[(long)s * cap + i]);
+  } else {
+    int c[3];
+    node_encode(k, p.x, p.y, p.z, c);
+    walk_cells(k, cstart, c, [&](int j) {
+      V4<T> q = pos[j];
+      T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
+      if (r2 > T(0) && r2 <= k.r2max) f(j);
+    });
+  }
+}
+template <class T, class F>
+__device__ __forceinline__ void for_bound_nbrs(const KP<T>& k, int i, int cap, int nbb,
+                                               const int* __restrict__ list,
+                                               const int* __restrict__ cstart,
+                                               const V4<T>* __restrict__ bpos, V4<T> p, F&& f) {
+  if (nbb <= k.list_cap_b) {
+    for (int s = 0; s < nbb; ++s) f(list[(long)s * cap + i]);
+  } else {
+    int c[3];
+    node_encode(k, p.x, p.y, p.z, c);
+    walk_cells(k, cstart, c, [&](int j) {
+      V4<T> q = bpos[j];
+      T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
+      if (r2 > T(0) && r2 <= k.r2max) f(j);
+    });
+  }
+}
+
+// Jacobi half-step P: IPPE_adv_psi_init + IPPE_adv_psi (f,f)(f,b) +
+// IPPE_psi_adv_non_negative (sph.py:196-226) for iteration kk of the solve.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_solver_psi(KP<T> k, int n, int cap, int kk, const V4<T>* __restrict__ fpos,
+                 const V4<T>* __restrict__ vadv, const T* __restrict__ psi_rho,
+                 const T* __restrict__ rdens, const T* __restrict__ alpha,
+                 const int* __restrict__ fstart, const V4<T>* __restrict__ bpos,
+                 const int* __restrict__ bstart, const int* __restrict__ nbr_f,
+                 const int* __restrict__ nbr_b, const int* __restrict__ nbr_nf,
+                 const int* __restrict__ nbr_nb, T* __restrict__ psi_adv, T* __restrict__ kappa,
+                 double* __restrict__ partial, TsphControl* ctrl) {
+  __shared__ double lds[kWaves];
+  __shared__ int flag;
+  if (ctrl->stop) return;
+  int nblk = gridDim.x;
+  int i = xcd_block(blockIdx.x, nblk) * kBlock + threadIdx.x;
+  const T dt = (T)ctrl->dt;
+  double err = 0;
+  if (i < n) {
+    V4<T> p = fpos[i];
+    V4<T> va = vadv[i];
+    T rp = rest_psi_of(k, rdens, i);
+    T psi;
+    {
+      TSPH_STRICT
+      psi = psi_rho[i] - rp;
+    }
+    for_fluid_nbrs(k, i, cap, nbr_nf[i], nbr_f, fstart, fpos, p, [&](int j) {
+      V4<T> q = fpos[j];
+      V4<T> vj = vadv[j];
+      T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+      T r2 = dx * dx + dy * dy + dz * dz, r;
+      T g = f_grad(k, r2, &r);
+      T dot = (dx * (va.x - vj.x) + dy * (va.y - vj.y) + dz * (va.z - vj.z)) * g;
+      psi += dot * q.w * dt;
+    });
+    for_bound_nbrs(k, i, cap, nbr_nb[i], nbr_b, bstart, bpos, p, [&](int j) {
+      V4<T> q = bpos[j];
+      T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+      T r2 = dx * dx + dy * dy + dz * dz, r;
+      T g = f_grad(k, r2, &r);
+      T dot = (dx * va.x + dy * va.y + dz * va.z) * g;  // boundary vel_adv is 0
+      psi += dot * q.w * dt;
+    });
+    if (psi < T(0)) psi = T(0);
+    psi_adv[i] = psi;
+    kappa[i] = psi / alpha[i];
+    err = (double)(psi / rp);
+  }
+  double s = block_sum(err, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  if (elect_last_block(&ctrl->counter, nblk, &flag)) {
+    double tot = sum_partials(partial, nblk, lds);
+    if (threadIdx.x == 0) {
+      TSPH_STRICT
+      T comp = (T)tot / (T)n;  // sph.py:226
+      ctrl->compression = (double)comp;
+      int solve = ctrl->solve;
+      if (kk < TSPH_MAX_ITER_LOG) {
+        if (solve == 0) ctrl->err_div[kk] = (double)comp;
+        else ctrl->err_incom[kk] = (double)comp;
+        ctrl->run_u[kk] = 1;
+      }
+      int it = kk + 1;
+      ctrl->it = it;
+      if (solve == 0) ctrl->div_iter = it;
+      else ctrl->incom_iter = it;
+      T thr = solve == 0 ? k.div_thr : k.comp_thr;
+      bool cont = (it < k.iter_min) || (comp > thr);
+      ctrl->stop = (it > k.iter_max || !cont) ? 1 : 0;
+    }
+  }
+}
+
+// Jacobi half-step U: IPPE_update_vel_adv (f,f)(f,b) (sph.py:229-244), with
+// kappa = psi_adv / alpha formed once per particle.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_solver_update(KP<T> k, int n, int cap, int kk, const V4<T>* __restrict__ fpos,
+                    const T* __restrict__ fmass, const T* __restrict__ kappa,
+                    const int* __restrict__ fstart, const V4<T>* __restrict__ bpos,
+                    const int* __restrict__ bstart, const int* __restrict__ nbr_f,
+                    const int* __restrict__ nbr_b, const int* __restrict__ nbr_nf,
+                    const int* __restrict__ nbr_nb, V4<T>* __restrict__ vadv,
+                    TsphControl* ctrl) {
+  if (kk >= TSPH_MAX_ITER_LOG || !ctrl->run_u[kk]) return;
+  int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const T dt = (T)ctrl->dt;
+  V4<T> p = fpos[i];
+  V4<T> va = vadv[i];
+  T ki = kappa[i];
+  T Xi = p.w;
+  T c = -(T(1) / dt) / fmass[i];
+  for_fluid_nbrs(k, i, cap, nbr_nf[i], nbr_f, fstart, fpos, p, [&](int j) {
+    V4<T> q = fpos[j];
+    T kj = kappa[j];
+    T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    T r2 = dx * dx + dy * dy + dz * dz, r;
+    T g = f_grad(k, r2, &r);
+    T s = c * (ki * q.w + kj * Xi) * g;
+    va.x += s * dx;
+    va.y += s * dy;
+    va.z += s * dz;
+  });
+  for_bound_nbrs(k, i, cap, nbr_nb[i], nbr_b, bstart, bpos, p, [&](int j) {
+    V4<T> q = bpos[j];
+    T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    T r2 = dx * dx + dy * dy + dz * dz, r;
+    T g = f_grad(k, r2, &r);
+    T s = c * (ki * q.w) * g;  // boundary psi_adv is 0
+    va.x += s * dx;
+    va.y += s * dy;
+    va.z += s * dz;
+  });
+  vadv[i] = va;
+}
+
+// ---------------------------------------------------------------------------
+// Advection: SPH_vel_adv_2_vel, gravity, viscosity (f,f), surface-tension
+// normals (f,f).  Neighbour velocities are read from vel_adv, so this sweep
+// must not write vel_adv: the update vel_adv = vel + dt * acce_adv happens in
+// the next kernel (k_surface_tension).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_advect(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
+             const T* __restrict__ vol, const T* __restrict__ rdens,
+             const int* __restrict__ fstart, const int* __restrict__ nbr_f,
+             const int* __restrict__ nbr_nf, V4<T>* __restrict__ vadv, V4<T>* __restrict__ vel,
+             V4<T>* __restrict__ acce_adv, V4<T>* __restrict__ normal) {
+  int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  V4<T> p = fpos[i];
+  V4<T> vi = vadv[i];
+  vel[i] = vi;  // SPH_vel_adv_2_vel
+  T ax = T(0) + k.g[0], ay = T(0) + k.g[1], az = T(0) + k.g[2];
+  T nx = 0, ny = 0, nz = 0;
+  const T fvis = k.two_d2 * k.mu / rdens[i];
+  for_fluid_nbrs(k, i, cap, nbr_nf[i], nbr_f, fstart, fpos, p, [&](int j) {
+    V4<T> q = fpos[j];
+    V4<T> vj = vadv[j];
+    T Vj = vol[j];
+    T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    T r2 = dx * dx + dy * dy + dz * dz, r;
+    T g = f_grad(k, r2, &r);
+    T Ax = (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz;
+    T s = fvis * Vj * g * Ax / (k.eps + r2);  // W_lap, sph_util.py:40-43
+    ax += s * dx;
+    ay += s * dy;
+    az += s * dz;
+    T t = -Vj * g;  // normal, sph.py:149
+    nx += t * dx;
+    ny += t * dy;
+    nz += t * dz;
+  });
+  acce_adv[i] = V4<T>{ax, ay, az, T(0)};
+  normal[i] = V4<T>{nx * k.h, ny * k.h, nz * k.h, T(0)};
+}
+
+// Surface tension loop 2 (sph.py:151-166) + SPH_advection_update_vel_adv
+// (sph.py:247-250).  The curvature term carries no kernel weight, so every
+// candidate of the neighbouring cells contributes: this sweep walks the cells.
+// With gamma == 0 every term is an exact zero and only the update runs.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_surface_tension(KP<T> k, int n, const V4<T>* __restrict__ fpos,
+                      const T* __restrict__ fmass, const T* __restrict__ psi_rho,
+                      const T* __restrict__ rdens, const T* __restrict__ vf,
+                      const int* __restrict__ fstart, const V4<T>* __restrict__ normal,
+                      const V4<T>* __restrict__ vel, V4<T>* __restrict__ acce_adv,
+                      V4<T>* __restrict__ vadv, const TsphControl* ctrl) {
+  TSPH_STRICT
+  int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const T dt = (T)ctrl->dt;
+  V4<T> a = acce_adv[i];
+  if (k.gamma != T(0) && vf[(long)i * k.P] > T(0.99)) {
+    V4<T> p = fpos[i];
+    V4<T> ni = normal[i];
+    T rp = rest_psi_of(k, rdens, i);
+    T psi_i = psi_rho[i];
+    int c[3];
+    node_encode(k, p.x, p.y, p.z, c);
+    walk_cells(k, fstart, c, [&](int j) {
+      V4<T> q = fpos[j];
+      T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+      T r = s_sqrt(s_r2(dx, dy, dz));
+      if (r > T(0) && vf[(long)j * k.P] > T(0.99)) {
+        T cs = -k.gamma * fmass[j] * s_C(k, r);
+        V4<T> nj = normal[j];
+        T f = T(2) * rp / (psi_i + psi_rho[j]);
+        a.x = a.x + f * (cs * dx / r + k.gamma * (ni.x - nj.x));
+        a.y = a.y + f * (cs * dy / r + k.gamma * (ni.y - nj.y));
+        a.z = a.z + f * (cs * dz / r + k.gamma * (ni.z - nj.z));
+      }
+    });
+    acce_adv[i] = a;
+  }
+  V4<T> v = vel[i];
+  vadv[i] = V4<T>{v.x + a.x * dt, v.y + a.y * dt, v.z + a.z * dt, T(0)};
+}
+
+// ---------------------------------------------------------------------------
+// FBM round (sph.py:301-372), predicated on ctrl->general_flag.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_fbm_drift(KP<T> k, int n, const V4<T>* __restrict__ vadv, const V4<T>* __restrict__ vel,
+                const T* __restrict__ rdens, const T* __restrict__ vf, V4<T>* __restrict__ acce_adv,
+                V4<T>* __restrict__ acce, T* __restrict__ zeta, V4<T>* __restrict__ drift,
+                T* __restrict__ tmp, TsphControl* ctrl) {
+  TSPH_STRICT
+  if (!ctrl->general_flag) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->fbm_rounds += 1;
+  int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const T dt = (T)ctrl->dt;
+  const int P = k.P;
+  for (int q = 0; q < P; ++q) tmp[(long)i * P + q] = T(0);  // SPH_FBM_clean_tmp
+  V4<T> va = vadv[i], v = vel[i];
+  T aa[3] = {(va.x - v.x) / dt, (va.y - v.y) / dt, (va.z - v.z) / dt};
+  T rd = rdens[i];
+  T z = T(0);
+  for (int q = 0; q < P; ++q) z = z + vf[(long)i * P + q] * (k.rho[q] - rd) / k.rho[q];
+  T fa[3];
+  for (int d = 0; d < 3; ++d) fa[d] = (aa[d] - z * k.g[d]) / (T(1) - z);
+  acce_adv[i] = V4<T>{aa[0], aa[1], aa[2], T(0)};
+  acce[i] = V4<T>{fa[0], fa[1], fa[2], T(0)};
+  zeta[i] = z;
+  for (int q = 0; q < P; ++q) {
+    T phi = vf[(long)i * P + q];
+    T s = phi * (k.rho[q] - rd);
+    T u[3];
+    for (int d = 0; d < 3; ++d) u[d] = s * (k.g[d] - fa[d]);
+    T wgt = phi * k.rho[q];
+    if (wgt > T(1e-6)) {
+      for (int d = 0; d < 3; ++d) u[d] = u[d] / wgt;
+    } else {
+      for (int d = 0; d < 3; ++d) u[d] = u[d] * T(0);
+    }
+    for (int d = 0; d < 3; ++d) u[d] = (u[d] + (fa[d] - aa[d])) * dt;
+    if (k.dim == 2) u[2] = T(0);
+    drift[(long)i * P + q] = V4<T>{u[0], u[1], u[2], T(0)};
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_fbm_flux(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
+               const T* __restrict__ rvol, const T* __restrict__ vf,
+               const V4<T>* __restrict__ drift, const int* __restrict__ flag,
+               const int* __restrict__ fstart, const int* __restrict__ nbr_f,
+               const int* __restrict__ nbr_nf, T* __restrict__ tmp, const TsphControl* ctrl) {
+  TSPH_STRICT
+  if (!ctrl->general_flag) return;
+  int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n || flag[i] != 0) return;
+  const T dt = (T)ctrl->dt;
+  const int P = k.P;
+  V4<T> p = fpos[i];
+  T acc[kMaxP];
+  for (int q = 0; q < P; ++q) acc[q] = tmp[(long)i * P + q];
+  const T pre_c = k.fbmC * dt;
+  const T pre_d = dt * k.fbmD;
+  int nf = nbr_nf[i];
+  // SPH_FBM_convect loop 2 (sph.py:346-363)
+  for_fluid_nbrs(k, i, cap, nf, nbr_f, fstart, fpos, p, [&](int j) {
+    if (flag[j] != 0) return;
+    V4<T> q = fpos[j];
+    T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    T r = s_sqrt(s_r2(dx, dy, dz));
+    T wg = s_Wg(k, r);
+    T ex = dx / r, ey = dy / r, ez = dz / r;
+    T Vj = rvol[j];
+    for (int s = 0; s < P; ++s) {
+      V4<T> ui = drift[(long)i * P + s], uj = drift[(long)j * P + s];
+      T pi = vf[(long)i * P + s], pj = vf[(long)j * P + s];
+      T fx = pi * ui.x + pj * uj.x, fy = pi * ui.y + pj * uj.y, fz = pi * ui.z + pj * uj.z;
+      T dot = fx * ex;
+      dot = dot + fy * ey;
+      dot = dot + fz * ez;
+      T t = pre_c * Vj * dot * wg;
+      acc[s] = acc[s] - t;
+    }
+  });
+  // SPH_FBM_diffuse (sph.py:308-326)
+  for_fluid_nbrs(k, i, cap, nf, nbr_f, fstart, fpos, p, [&](int j) {
+    if (flag[j] != 0) return;
+    V4<T> q = fpos[j];
+    T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    T r2 = s_r2(dx, dy, dz);
+    T r = s_sqrt(r2);
+    T wg = s_Wg(k, r);
+    T den = r * r + k.eps;
+    T Vj = rvol[j];
+    for (int s = 0; s < P; ++s) {
+      T dphi = vf[(long)i * P + s] - vf[(long)j * P + s];
+      T t = pre_d * dphi * Vj * r * wg / den;
+      acc[s] = acc[s] + t;
+    }
+  });
+  for (int q = 0; q < P; ++q) tmp[(long)i * P + q] = acc[q];
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_fbm_check(KP<T> k, int n, const T* __restrict__ vf, const T* __restrict__ tmp,
+                int* __restrict__ flag, double* __restrict__ partial, TsphControl* ctrl) {
+  TSPH_STRICT
+  __shared__ double lds[kWaves];
+  __shared__ int lflag;
+  if (!ctrl->general_flag) return;
+  int nblk = gridDim.x;
+  int i = xcd_block(blockIdx.x, nblk) * kBlock + threadIdx.x;
+  double any = 0;
+  if (i < n) {
+    bool neg = false;  // has_negative, sph_util.py:56-62
+    for (int q = 0; q < k.P; ++q)
+      if (vf[(long)i * k.P + q] + tmp[(long)i * k.P + q] < T(0)) neg = true;
+    if (neg) {
+      flag[i] = 1;
+      any = 1;
+    }
+  }
+  double s = block_sum(any, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  if (elect_last_block(&ctrl->counter, nblk, &lflag)) {
+    double tot = sum_partials(partial, nblk, lds);
+    if (threadIdx.x == 0) ctrl->general_flag = tot > 0 ? 1 : 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tail: [SPH_update_volume_frac], SPH_update_mass, SPH_update_pos,
+// SPH_update_energy (sph.py:265-286, 375-379).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_tail(KP<T> k, int n, int fbm, V4<T>* __restrict__ pos, V4<T>* __restrict__ vel,
+           const V4<T>* __restrict__ vadv, T* __restrict__ mass, T* __restrict__ rdens,
+           const T* __restrict__ rvol, T* __restrict__ vf, const T* __restrict__ tmp,
+           const int* __restrict__ flag, double* __restrict__ partial, TsphControl* ctrl) {
+  TSPH_STRICT
+  __shared__ double lds[kWaves];
+  __shared__ int lflag;
+  int nblk = gridDim.x;
+  int i = xcd_block(blockIdx.x, nblk) * kBlock + threadIdx.x;
+  const T dt = (T)ctrl->dt;
+  double ke = 0, pe = 0;
+  if (i < n) {
+    const int P = k.P;
+    if (fbm && !(flag[i] > 0))
+      for (int q = 0; q < P; ++q) vf[(long)i * P + q] = vf[(long)i * P + q] + tmp[(long)i * P + q];
+    T rd = k.rho[0] * vf[(long)i * P];
+    for (int q = 1; q < P; ++q) rd = rd + k.rho[q] * vf[(long)i * P + q];
+    T m = rd * rvol[i];
+    rdens[i] = rd;
+    mass[i] = m;
+    V4<T> v = vadv[i];
+    V4<T> p = pos[i];
+    p.x = p.x + v.x * dt;
+    p.y = p.y + v.y * dt;
+    p.z = p.z + v.z * dt;
+    p.w = k.solver == TSPH_VFSPH ? rvol[i] : m;
+    pos[i] = p;
+    vel[i] = V4<T>{v.x, v.y, v.z, T(0)};
+    T n2 = s_r2(v.x, v.y, v.z);
+    ke = (double)(T(0.5) * m * n2);
+    pe = (double)(-m * k.g[1] * (p.y - k.lb[1]));
+  }
+  double s1 = block_sum(ke, lds);
+  double s2 = block_sum(pe, lds);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = s1;
+    partial[nblk + blockIdx.x] = s2;
+  }
+  if (elect_last_block(&ctrl->counter, nblk, &lflag)) {
+    double a = sum_partials(partial, nblk, lds);
+    double b = sum_partials(partial + nblk, nblk, lds);
+    if (threadIdx.x == 0) {
+      ctrl->ke = a;
+      ctrl->pe = b;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+// ---------------------------------------------------------------------------
+template <class T>
+int density_alpha(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl* ctrl,
+                  void* ws, size_t wsb, hipStream_t st) {
+  KP<T> k = make_kp<T>(prm);
+  hipLaunchKernelGGL(k_step_reset, dim3(1), dim3(64), 0, st, ctrl);
+  if (f->n > 0)
+    hipLaunchKernelGGL(k_pass_a_fluid<T>, dim3(blocks_for(f->n)), dim3(kBlock), 0, st, k, f->n,
+                       f->capacity, (const V4<T>*)f->pos, (const T*)f->mass,
+                       (const T*)f->rest_volume, f->cell_start, b->n, (const V4<T>*)b->pos,
+                       (const T*)b->mass, (const T*)b->rest_volume, b->cell_start, (T*)f->W,
+                       (T*)f->sph_compression, (T*)f->sph_density, (V4<T>*)f->alpha_1,
+                       (T*)f->alpha_2, (T*)f->alpha, (T*)f->vol, (const V4<T>*)f->vel,
+                       (V4<T>*)f->vel_adv, f->nbr_f, f->nbr_b, f->nbr_nf, f->nbr_nb, f->flag, ctrl);
+  if (b->n > 0)
+    hipLaunchKernelGGL(k_pass_a_bound<T>, dim3(blocks_for(b->n)), dim3(kBlock), 0, st, k, b->n,
+                       (const V4<T>*)b->pos, (const T*)b->mass, (const T*)b->rest_volume,
+                       b->cell_start, (const V4<T>*)f->pos, (const T*)f->mass,
+                       (const T*)f->rest_volume, f->cell_start, (T*)b->W, (T*)b->sph_compression,
+                       (T*)b->sph_density, (V4<T>*)b->alpha_1, (T*)b->alpha_2, (T*)b->alpha,
+                       (T*)b->vol);
+  TSPH_LAUNCH_CHECK("density_alpha");
+  return 0;
+}
+
+template <class T>
+int solver_begin(const TsphParams& prm, TsphControl* ctrl, int solve, hipStream_t st) {
+  KP<T> k = make_kp<T>(prm);
+  hipLaunchKernelGGL(k_solver_begin<T>, dim3(1), dim3(kBlock), 0, st, k, ctrl, solve);
+  TSPH_LAUNCH_CHECK("solver_begin");
+  return 0;
+}
+
+template <class T>
+int solver_iteration(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl* ctrl,
+                     int kk, void* ws, size_t wsb, hipStream_t st) {
+  KP<T> k = make_kp<T>(prm);
+  WsLayout L;
+  TSPH_REQUIRE(carve_ws(ws, wsb, f->capacity, prm.node_num, &L), TSPH_ECAPACITY,
+               "tsph_solver_iteration: workspace too small");
+  TSPH_REQUIRE(f->n > 0, TSPH_EINVAL, "tsph_solver_iteration: no fluid particles");
+  int nb = blocks_for(f->n);
+  const T* psi = (const T*)(prm.solver_type == TSPH_VFSPH ? f->sph_compression : f->sph_density);
+  hipLaunchKernelGGL(k_solver_psi<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity, kk,
+                     (const V4<T>*)f->pos, (const V4<T>*)f->vel_adv, psi,
+                     (const T*)f->rest_density, (const T*)f->alpha, f->cell_start,
+                     (const V4<T>*)b->pos, b->cell_start, f->nbr_f, f->nbr_b, f->nbr_nf,
+                     f->nbr_nb, (T*)f->psi_adv, (T*)f->kappa, L.partial, ctrl);
+  hipLaunchKernelGGL(k_solver_update<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity, kk,
+                     (const V4<T>*)f->pos, (const T*)f->mass, (const T*)f->kappa, f->cell_start,
+                     (const V4<T>*)b->pos, b->cell_start, f->nbr_f, f->nbr_b, f->nbr_nf,
+                     f->nbr_nb, (V4<T>*)f->vel_adv, ctrl);
+  TSPH_LAUNCH_CHECK("solver_iteration");
+  return 0;
+}
+
+template <class T>
+int advect(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl* ctrl, void* ws,
+           size_t wsb, hipStream_t st) {
+  KP<T> k = make_kp<T>(prm);
+  if (f->n == 0) return 0;
+  int nb = blocks_for(f->n);
+  hipLaunchKernelGGL(k_advect<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
+                     (const V4<T>*)f->pos, (const T*)f->vol, (const T*)f->rest_density,
+                     f->cell_start, f->nbr_f, f->nbr_nf, (V4<T>*)f->vel_adv, (V4<T>*)f->vel,
+                     (V4<T>*)f->acce_adv, (V4<T>*)f->normal);
+  const T* psi = (const T*)(prm.solver_type == TSPH_VFSPH ? f->sph_compression : f->sph_density);
+  hipLaunchKernelGGL(k_surface_tension<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n,
+                     (const V4<T>*)f->pos, (const T*)f->mass, psi, (const T*)f->rest_density,
+                     (const T*)f->volume_frac, f->cell_start, (const V4<T>*)f->normal,
+                     (const V4<T>*)f->vel, (V4<T>*)f->acce_adv, (V4<T>*)f->vel_adv, ctrl);
+  TSPH_LAUNCH_CHECK("advect");
+  return 0;
+}
+
+template <class T>
+int fbm_round(const TsphParams& prm, TsphObject* f, TsphControl* ctrl, void* ws, size_t wsb,
+              hipStream_t st) {
+  KP<T> k = make_kp<T>(prm);
+  WsLayout L;
+  TSPH_REQUIRE(carve_ws(ws, wsb, f->capacity, prm.node_num, &L), TSPH_ECAPACITY,
+               "tsph_fbm_round: workspace too small");
+  if (f->n == 0) return 0;
+  int nb = blocks_for(f->n);
+  hipLaunchKernelGGL(k_fbm_drift<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n,
+                     (const V4<T>*)f->vel_adv, (const V4<T>*)f->vel, (const T*)f->rest_density,
+                     (const T*)f->volume_frac, (V4<T>*)f->acce_adv, (V4<T>*)f->acce,
+                     (T*)f->fbm_zeta, (V4<T>*)f->drift_vel, (T*)f->volume_frac_tmp, ctrl);
+  hipLaunchKernelGGL(k_fbm_flux<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
+                     (const V4<T>*)f->pos, (const T*)f->rest_volume, (const T*)f->volume_frac,
+                     (const V4<T>*)f->drift_vel, f->flag, f->cell_start, f->nbr_f, f->nbr_nf,
+                     (T*)f->volume_frac_tmp, ctrl);
+  hipLaunchKernelGGL(k_fbm_check<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n,
+                     (const T*)f->volume_frac, (const T*)f->volume_frac_tmp, f->flag, L.partial,
+                     ctrl);
+  TSPH_LAUNCH_CHECK("fbm_round");
+  return 0;
+}
+
+template <class T>
+int tail(const TsphParams& prm, TsphObject* f, TsphControl* ctrl, int fbm, void* ws, size_t wsb,
+         hipStream_t st) {
+  KP<T> k = make_kp<T>(prm);
+  WsLayout L;
+  TSPH_REQUIRE(carve_ws(ws, wsb, f->capacity, prm.node_num, &L), TSPH_ECAPACITY,
+               "tsph_tail: workspace too small");
+  int nb = blocks_for(f->n > 0 ? f->n : 1);
+  hipLaunchKernelGGL(k_tail<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, fbm, (V4<T>*)f->pos,
+                     (V4<T>*)f->vel, (const V4<T>*)f->vel_adv, (T*)f->mass,
+                     (T*)f->rest_density, (const T*)f->rest_volume, (T*)f->volume_frac,
+                     (const T*)f->volume_frac_tmp, f->flag, L.partial, ctrl);
+  TSPH_LAUNCH_CHECK("tail");
+  return 0;
+}
+
+template int density_alpha<float>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
+template int density_alpha<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
+template int solver_begin<float>(const TsphParams&, TsphControl*, int, hipStream_t);
+template int solver_begin<double>(const TsphParams&, TsphControl*, int, hipStream_t);
+template int solver_iteration<float>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, void*, size_t, hipStream_t);
+template int solver_iteration<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, void*, size_t, hipStream_t);
+template int advect<float>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
+template int advect<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
+template int fbm_round<float>(const TsphParams&, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
+template int fbm_round<double>(const TsphParams&, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
+template int tail<float>(const TsphParams&, TsphObject*, TsphControl*, int, void*, size_t, hipStream_t);
+template int tail<double>(const TsphParams&, TsphObject*, TsphControl*, int, void*, size_t, hipStream_t);
+
+}  // namespace tsph

@@ -1,0 +1,289 @@
+"""The per-step driver: the reference's `sph_step()` (main.py:175-256).
+
+`Simulation.sph_step()` is the fused MI355X path: one counting sort, one
+pass that computes density / alpha and records neighbour lists, predicated
+Jacobi iterations whose stop state lives on the device, advection, optional
+FBM rounds and the tail.  The host only reads the 8-byte stop state once per
+launched chunk of iterations (the reference reads `fluid.compression` after
+every iteration, main.py:201 / 222).
+
+`Simulation.sph_step_reference()` issues the reference's kernels one by one
+in main.py's order through the kernel surface in `sph.py`; it exists for
+drop-in use and as a parity anchor.
+"""
+import ctypes
+import json
+
+import numpy as np
+import torch
+
+from . import _lib
+from .config import Config
+from .sph_obj import Fluid, Ngrid
+
+
+class StepStats:
+    def __init__(self, raw, precision):
+        self.div_iter = raw.div_iter
+        self.incom_iter = raw.incom_iter
+        self.fbm_rounds = raw.fbm_rounds
+        self.dt = raw.dt
+        self.compression = raw.compression
+        self.kinetic_energy = raw.ke
+        self.gravity_potential_energy = raw.pe
+        self.div_err = list(raw.err_div[: min(raw.div_iter, _lib.TSPH_MAX_ITER_LOG)])
+        self.incom_err = list(raw.err_incom[: min(raw.incom_iter, _lib.TSPH_MAX_ITER_LOG)])
+        self.overflow_f = raw.overflow_f
+        self.overflow_b = raw.overflow_b
+
+    def __repr__(self):
+        return (f"StepStats(div_iter={self.div_iter}, incom_iter={self.incom_iter}, dt={self.dt:.6g}, "
+                f"compression={self.compression:.4g})")
+
+
+class Simulation:
+    """Owns the two particle objects (fluid uid 0, boundary uid 1), the
+    control block, the workspace and the reference-layout Ngrid."""
+
+    def __init__(self, config, device="cuda", precision=4, fbm=False, wcsph=False, cfl=True,
+                 list_cap_f=64, list_cap_b=48, fluid_max=None, bound_max=None):
+        self.config = config
+        self.device = torch.device(device)
+        self.precision = precision
+        self.lib = _lib.load()
+        self.fbm = fbm
+        self.cfl = cfl
+        self.params = config.params(precision, list_cap_f, list_cap_b)
+        fm = config.fluid_max_part_num if fluid_max is None else fluid_max
+        bm = config.bound_max_part_num if bound_max is None else bound_max
+        self.fluid = Fluid(config, fm, 0, device, precision, list_cap_f, list_cap_b)
+        self.bound = Fluid(config, bm, 1, device, precision, list_cap_f, list_cap_b)
+        cap = max(fm, 1) + max(bm, 1)
+        self.ws = torch.zeros(int(self.lib.tsph_workspace_bytes(ctypes.byref(self.params), cap)),
+                              dtype=torch.uint8, device=self.device)
+        self.ctrl = torch.zeros(int(self.lib.tsph_control_bytes()), dtype=torch.uint8, device=self.device)
+        self._ctrl_host = torch.zeros(self.ctrl.numel(), dtype=torch.uint8).pin_memory()
+        self._stop_host = torch.zeros(16, dtype=torch.uint8).pin_memory()
+        self.ngrid = Ngrid(self)
+        self.bound_dirty = True
+        self.pred = [max(config.iter_threshold_min, 1), max(config.iter_threshold_min, 1)]
+        if config.iter_threshold_max + 1 > _lib.TSPH_MAX_ITER_LOG:
+            raise Exception(f"solver_iter_threshold_max must be < {_lib.TSPH_MAX_ITER_LOG}")
+        self.set_dt(config.dt)
+        self.time_count = 0.0
+
+    # ----------------------------------------------------------------- helpers
+    def stream_ptr(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def ctrl_ptr(self):
+        return ctypes.c_void_p(self.ctrl.data_ptr())
+
+    def _ws(self):
+        return ctypes.c_void_p(self.ws.data_ptr()), self.ws.numel()
+
+    def control(self):
+        """The control block (include/tichi_sph.h TsphControl), synchronising."""
+        self._ctrl_host.copy_(self.ctrl, non_blocking=True)
+        torch.cuda.current_stream(self.device).synchronize()
+        return _lib.TsphControl.from_buffer_copy(self._ctrl_host.numpy().tobytes())
+
+    def set_dt(self, dt):
+        raw = self.control() if self.ctrl.is_cuda else _lib.TsphControl()
+        raw.dt = dt
+        self._write_ctrl(raw)
+
+    def set_compression(self, c):
+        raw = self.control()
+        raw.compression = c
+        self._write_ctrl(raw)
+
+    def _write_ctrl(self, raw):
+        buf = torch.frombuffer(bytearray(bytes(raw)), dtype=torch.uint8)
+        self.ctrl.copy_(buf.to(self.device))
+
+    def _stop_state(self):
+        # it, stop are the first two int32 after two doubles
+        self._stop_host.copy_(self.ctrl[16:32], non_blocking=True)
+        torch.cuda.current_stream(self.device).synchronize()
+        a = self._stop_host.numpy().view(np.int32)
+        return int(a[0]), int(a[1]), int(a[3])  # it, stop, general_flag
+
+    # --------------------------------------------------------------- scenario
+    def add_to_scenario(self, obj_str, param):
+        """main.py:42-60 for the `cube` and `box` object types."""
+        obj = self.fluid if obj_str == "fluid" else self.bound if obj_str == "bound" else None
+        if obj is None:
+            print("scenario WARNING: obj '", obj_str, "' non-exist.")
+            return
+        color = int(param.get("color", "0x0"), 16)
+        if param["type"] == "cube":
+            obj.scene_add_cube(param["start_pos"], param["end_pos"], param["volume_frac"], param["vel"],
+                               color, param["particle_relaxing_factor"])
+        elif param["type"] == "box":
+            obj.scene_add_box(param["start_pos"], param["end_pos"], param["layers"], param["volume_frac"],
+                              param["vel"], color, param["particle_relaxing_factor"])
+        else:
+            raise Exception("scenario ERROR: object type '%s' is not supported by this build" % param["type"])
+        if obj is self.bound:
+            self.bound_dirty = True
+
+    def init_scenario(self):
+        """main.py:62-79: objects without a positive `time` are added now."""
+        sb = self.config.scenario_buffer
+        for part in sb:
+            if part == "sim_env":
+                continue
+            for param in sb[part]["objs"]:
+                if "time" not in param or param["time"] <= 0.0:
+                    self.add_to_scenario(part, param)
+
+    # ------------------------------------------------------------ fused step
+    def _sort_bound(self):
+        P = ctypes.byref(self.params)
+        ws, wsb = self._ws()
+        rc = self.lib.tsph_sort_object(P, ctypes.byref(self.bound.struct()), self.ctrl_ptr(), 0, ws, wsb,
+                                       self.stream_ptr())
+        _lib.check(rc, "tsph_sort_object(bound)")
+        self.bound.adopt_swaps()
+        self.bound_dirty = False
+
+    def _solve(self, solve):
+        P = ctypes.byref(self.params)
+        ws, wsb = self._ws()
+        st = self.stream_ptr()
+        f, b = ctypes.byref(self.fluid.struct()), ctypes.byref(self.bound.struct())
+        _lib.check(self.lib.tsph_solver_begin(P, self.ctrl_ptr(), solve, st), "tsph_solver_begin")
+        kmax = self.config.iter_threshold_max + 1
+        k = 0
+        chunk = max(self.pred[solve], 1)
+        while k < kmax:
+            for _ in range(min(chunk, kmax - k)):
+                rc = self.lib.tsph_solver_iteration(P, f, b, self.ctrl_ptr(), k, ws, wsb, st)
+                _lib.check(rc, "tsph_solver_iteration")
+                k += 1
+            it, stop, _ = self._stop_state()
+            if stop:
+                break
+            chunk = 1
+        self.pred[solve] = max(it, 1)
+        return it
+
+    def sph_step(self, stats=False):
+        """One step of main.py:175-256 (preceded by cfl_condition, main.py:348)."""
+        if self.fluid.part_num == 0:
+            raise Exception("sph_step: no fluid particles")
+        P = ctypes.byref(self.params)
+        ws, wsb = self._ws()
+        st = self.stream_ptr()
+        if self.bound_dirty:
+            self._sort_bound()
+        rc = self.lib.tsph_sort_object(P, ctypes.byref(self.fluid.struct()), self.ctrl_ptr(),
+                                       1 if self.cfl else 0, ws, wsb, st)
+        _lib.check(rc, "tsph_sort_object(fluid)")
+        self.fluid.adopt_swaps()
+        f, b = ctypes.byref(self.fluid.struct()), ctypes.byref(self.bound.struct())
+        _lib.check(self.lib.tsph_density_alpha(P, f, b, self.ctrl_ptr(), ws, wsb, st), "tsph_density_alpha")
+        div_it = self._solve(0)
+        _lib.check(self.lib.tsph_advect(P, f, b, self.ctrl_ptr(), ws, wsb, st), "tsph_advect")
+        inc_it = self._solve(1)
+        if self.fbm:
+            rounds = 0
+            while rounds < 1000:
+                _lib.check(self.lib.tsph_fbm_round(P, f, self.ctrl_ptr(), ws, wsb, st), "tsph_fbm_round")
+                rounds += 1
+                _, _, gflag = self._stop_state()
+                if not gflag:
+                    break
+        _lib.check(self.lib.tsph_tail(P, f, self.ctrl_ptr(), 1 if self.fbm else 0, ws, wsb, st), "tsph_tail")
+        if stats:
+            return StepStats(self.control(), self.precision)
+        return div_it, inc_it
+
+    # -------------------------------------------------- reference-order step
+    def sph_step_reference(self, wcsph=False, fbm=None):
+        """main.py:175-256 kernel by kernel through the sph.py surface
+        (cfl_condition first, as run_step does at main.py:348)."""
+        from . import sph as K
+
+        fbm = self.fbm if fbm is None else fbm
+        fluid, bound, ngrid = self.fluid, self.bound, self.ngrid
+        cfg = self.config
+        if self.cfl:
+            K.cfl_condition(self, fluid)
+        ngrid.clear_node()
+        ngrid.encode(fluid)
+        ngrid.encode(bound)
+        ngrid.mem_shift()
+        ngrid.fill_node(fluid)
+        ngrid.fill_node(bound)
+        K.SPH_clean_value(self, fluid)
+        K.SPH_clean_value(self, bound)
+        K.SPH_prepare_attr(self, fluid, fluid)
+        K.SPH_prepare_attr(self, fluid, bound)
+        K.SPH_prepare_attr(self, bound, bound)
+        K.SPH_prepare_attr(self, bound, fluid)
+        K.SPH_prepare_alpha_1(self, fluid, fluid)
+        K.SPH_prepare_alpha_1(self, fluid, bound)
+        K.SPH_prepare_alpha_2(self, fluid, fluid)
+        K.SPH_prepare_alpha_2(self, bound, fluid)
+        K.SPH_prepare_alpha(self, fluid)
+        K.SPH_prepare_alpha(self, bound)
+        out = {}
+        for solve, thr in ((0, cfg.divergence_threshold), (1, cfg.compression_threshold)):
+            it = 0
+            errs = []
+            if solve == 0:
+                K.SPH_vel_2_vel_adv(self, fluid)
+            while it < cfg.iter_threshold_min or self.compression() > thr:
+                K.IPPE_adv_psi_init(self, fluid)
+                K.IPPE_adv_psi(self, fluid, fluid)
+                K.IPPE_adv_psi(self, fluid, bound)
+                K.IPPE_psi_adv_non_negative(self, fluid)
+                errs.append(self.compression())
+                K.IPPE_update_vel_adv(self, fluid, fluid)
+                K.IPPE_update_vel_adv(self, fluid, bound)
+                it += 1
+                if it > cfg.iter_threshold_max:
+                    break
+            out[solve] = (it, errs)
+            if solve == 0:
+                K.SPH_vel_adv_2_vel(self, fluid)
+                K.SPH_advection_gravity_acc(self, fluid)
+                K.SPH_advection_viscosity_acc(self, fluid, fluid)
+                K.SPH_advection_surface_tension_acc(self, fluid, fluid)
+                K.SPH_advection_update_vel_adv(self, fluid)
+        if wcsph:
+            K.WC_pressure_val(self, fluid)
+            K.WC_pressure_acce(self, fluid, fluid)
+            K.WC_pressure_acce(self, fluid, bound)
+            K.SPH_advection_update_vel_adv(self, fluid)
+        rounds = 0
+        if fbm:
+            while self.control().general_flag > 0 and rounds < 1000:
+                K.SPH_FBM_clean_tmp(self, fluid)
+                K.SPH_FBM_convect(self, fluid, fluid)
+                K.SPH_FBM_diffuse(self, fluid, fluid)
+                K.SPH_FBM_check_tmp(self, fluid)
+                rounds += 1
+            K.SPH_update_volume_frac(self, fluid)
+        K.SPH_update_mass(self, fluid)
+        K.SPH_update_pos(self, fluid)
+        K.SPH_update_energy(self, fluid)
+        return out[0][0], out[1][0], out[0][1], out[1][1], rounds
+
+    def compression(self):
+        c = self.control().compression
+        return float(np.float32(c)) if self.precision == 4 else float(c)
+
+
+def load_simulation(config_file, scenario_file, **kw):
+    sim = Simulation(Config.from_files(config_file, scenario_file), **kw)
+    sim.init_scenario()
+    return sim
+
+
+def simulation_from_dicts(config_buffer, scenario_buffer, **kw):
+    sim = Simulation(Config(json.loads(json.dumps(config_buffer)), json.loads(json.dumps(scenario_buffer))), **kw)
+    sim.init_scenario()
+    return sim

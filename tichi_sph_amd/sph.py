@@ -1,0 +1,56 @@
+"""The reference's kernel surface (sph.py): one function per Taichi kernel,
+same names and (ngrid, obj, nobj) argument meaning, each enqueuing the
+matching HIP kernel through tsph_ref_kernel.  The grid is the Simulation's
+reference-layout Ngrid (build it with ngrid.clear_node / encode / mem_shift /
+fill_node first, as main.py:178-183 does).
+
+The first argument is the Simulation that owns the objects (the reference
+reaches the same state through module globals).
+"""
+import ctypes
+
+from . import _lib
+
+
+def _run(sim, name, obj, nobj=None):
+    P = ctypes.byref(sim.params)
+    ws = ctypes.c_void_p(sim.ws.data_ptr())
+    npc, nps, pin, uin = sim.ngrid.ptrs()
+    rc = sim.lib.tsph_ref_kernel(P, _lib.KERNEL_IDS[name], ctypes.byref(obj.struct()),
+                                 ctypes.byref(nobj.struct()) if nobj is not None else None,
+                                 npc, nps, pin, uin, sim.ctrl_ptr(), ws, sim.ws.numel(),
+                                 sim.stream_ptr())
+    _lib.check(rc, name)
+
+
+def _make_obj_kernel(name):
+    def fn(sim, obj):
+        _run(sim, name, obj)
+    fn.__name__ = name
+    fn.__doc__ = f"{name}(obj): reference kernel of the same name (sph.py)."
+    return fn
+
+
+def _make_pair_kernel(name):
+    def fn(sim, obj, nobj):
+        _run(sim, name, obj, nobj)
+    fn.__name__ = name
+    fn.__doc__ = f"{name}(ngrid, obj, nobj): reference kernel of the same name (sph.py)."
+    return fn
+
+
+_OBJ = ["SPH_clean_value", "cfl_condition", "SPH_prepare_alpha", "SPH_advection_gravity_acc",
+        "WC_pressure_val", "IPPE_adv_psi_init", "IPPE_psi_adv_non_negative",
+        "SPH_advection_update_vel_adv", "SPH_vel_2_vel_adv", "SPH_vel_adv_2_vel", "SPH_update_pos",
+        "SPH_update_energy", "SPH_update_mass", "SPH_FBM_clean_tmp", "SPH_FBM_check_tmp",
+        "SPH_update_volume_frac"]
+_PAIR = ["SPH_prepare_attr", "SPH_prepare_alpha_1", "SPH_prepare_alpha_2",
+         "SPH_advection_viscosity_acc", "SPH_advection_surface_tension_acc", "WC_pressure_acce",
+         "IPPE_adv_psi", "IPPE_update_vel_adv", "SPH_FBM_diffuse", "SPH_FBM_convect"]
+
+for _n in _OBJ:
+    globals()[_n] = _make_obj_kernel(_n)
+for _n in _PAIR:
+    globals()[_n] = _make_pair_kernel(_n)
+
+__all__ = _OBJ + _PAIR
