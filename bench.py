@@ -1,0 +1,149 @@
+"""Benchmark: DFSPH particle-steps/s on the 1M-particle 3D dam break.
+
+One step = one fused sph_step (neighbour sort, density/alpha pass, both
+Jacobi solves to their stop rule, advection, tail) over the 1,000,000-fluid /
+206,464-boundary particle scene (tichi_sph_amd/scenarios/dambreak_3d_1m),
+inputs resident in HBM.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Roofline: algorithmic bytes of the dominant kernel per launch (per-particle
+figures of the minimal-pass schedule) divided by its average launch time from
+HIP events recorded on the launch stream inside the timed region.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "DFSPH particle-steps/s @1M particles, fwd vs fwd+adjoint; % HBM peak"
+WORKLOAD = "dambreak_3d_1m"
+
+# Algorithmic bytes per particle and launch (FP32, 3D; s = 4 B, v = 16 B with
+# the 4th lane counted as the vector's padding-free 12 B + X where it carries X).
+# fluid: P reads pos, vel_adv, X, psi, psi0 and writes psi_adv (2v + 4s);
+# U reads pos, vel_adv, psi_adv, alpha, X and writes vel_adv (3v + 3s);
+# A reads pos, X and writes psi, alpha (v + 3s).  Boundary particles: pos and
+# X per pass.  v = 12 B, s = 4 B.
+V, S = 12, 4
+ALG_BYTES = {
+    "solver_psi": lambda nf, nb: nf * (2 * V + 4 * S) + nb * (V + S),
+    "solver_update": lambda nf, nb: nf * (3 * V + 3 * S) + nb * (V + S),
+    "density_alpha": lambda nf, nb: nf * (V + 3 * S) + nb * (2 * V + 4 * S),
+    "advect": lambda nf, nb: nf * (3 * V + 3 * S),
+}
+
+
+def step_bytes(nf, nb, nc, I):
+    """B_step of the minimal-pass schedule (3D, FP32)."""
+    return nf * ((10 * V + 11 * S) + I * (5 * V + 7 * S)) + nb * (2 * V + 4 * S) * (1 + I) + 16 * nc
+
+
+def cpu_baseline(steps=2):
+    """The oracle (C++ restatement of the reference step, reference schedule:
+    atomic bucket grid each step, unsorted storage, separate (obj, nobj)
+    sweeps) on this host's cores, on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    cfg = json.load(open(os.path.join(ROOT, "tichi_sph_amd", "scenarios", WORKLOAD + ".config.json")))
+    scn = json.load(open(os.path.join(ROOT, "tichi_sph_amd", "scenarios", WORKLOAD + ".json")))
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+    O.lib().oracle_set_threads(cores)
+    o = O.Oracle(cfg, scn, 4)
+    o.init_scenario()
+    o.step(O.STEP_CFL)  # warm-up
+    t0 = time.perf_counter()
+    its = []
+    for _ in range(steps):
+        st = o.step(O.STEP_CFL)
+        its.append((st.div_iter, st.incom_iter))
+    t = time.perf_counter() - t0
+    O.lib().oracle_set_threads(1)
+    nf = o.n[0]
+    return {"value": nf * steps / t, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{WORKLOAD}: {steps} steps after 1 warm-up step ({nf} fluid particles), "
+                      f"{t:.1f} s wall, iterations {its}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-timers", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (--gpus must be 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from tichi_sph_amd import load_simulation
+
+    scen = os.path.join(ROOT, "tichi_sph_amd", "scenarios")
+    sim = load_simulation(os.path.join(scen, WORKLOAD + ".config.json"), os.path.join(scen, WORKLOAD + ".json"))
+    nf, nb, nc = sim.fluid.part_num, sim.bound.part_num, sim.config.node_num
+    for _ in range(args.warmup):
+        sim.sph_step()
+    torch.cuda.synchronize()
+    sim.timer = None if args.no_timers else []
+    iters = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        iters.append(sim.sph_step())
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall / args.steps * 1e3
+    value = nf * args.steps / wall
+
+    per = {}
+    if sim.timer:
+        for name, e0, e1 in sim.timer:
+            d = per.setdefault(name, [0.0, 0])
+            d[0] += e0.elapsed_time(e1)
+            d[1] += 1
+    I_mean = sum(a + b for a, b in iters) / len(iters)
+    roof = None
+    if per:
+        dom = max((k for k in per if k in ALG_BYTES), key=lambda k: per[k][0])
+        avg_ms = per[dom][0] / per[dom][1]
+        alg = ALG_BYTES[dom](nf, nb)
+        ach = alg / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "avg_launch_us": round(avg_ms * 1e3, 2), "alg_bytes_per_launch": alg}
+    B = step_bytes(nf, nb, nc, I_mean)
+    step_gbs = B / (ms_per_step * 1e-3) / 1e9
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (lattice dam break generated from the scenario JSON)",
+        "config": {"workload": WORKLOAD, "fluid_particles": nf, "boundary_particles": nb, "cells": nc,
+                   "solver": "DFSPH", "dt": sim.config.dt, "parallelism": "single GPU"},
+        "roofline": roof,
+        "step_roofline": {"alg_bytes_per_step": int(B), "achieved_GBs": round(step_gbs, 1),
+                          "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+        "iterations": {"mean_div": sum(a for a, _ in iters) / len(iters),
+                       "mean_incom": sum(b for _, b in iters) / len(iters),
+                       "max_div": max(a for a, _ in iters), "max_incom": max(b for _, b in iters)},
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in sorted(per.items())},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -194,6 +194,16 @@ int tsph_solver_iteration(const TsphParams* p, TsphObject* fluid, TsphObject* bo
                           TsphControl* ctrl, int32_t k, void* workspace, size_t ws_bytes,
                           void* stream);
 
+/* The two halves of tsph_solver_iteration as separate launches:
+ * psi = IPPE_adv_psi_init + IPPE_adv_psi (f,f)(f,b) + IPPE_psi_adv_non_negative
+ * (sph.py:196-226) with the stop decision; update = IPPE_update_vel_adv
+ * (f,f)(f,b) (sph.py:229-244). */
+int tsph_solver_psi(const TsphParams* p, TsphObject* fluid, TsphObject* bound, TsphControl* ctrl,
+                    int32_t k, void* workspace, size_t ws_bytes, void* stream);
+int tsph_solver_update(const TsphParams* p, TsphObject* fluid, TsphObject* bound,
+                       TsphControl* ctrl, int32_t k, void* workspace, size_t ws_bytes,
+                       void* stream);
+
 /*
  * SPH_vel_adv_2_vel + SPH_advection_gravity_acc + SPH_advection_viscosity_acc
  * (f,f) + SPH_advection_surface_tension_acc (f,f) + SPH_advection_update_vel_adv

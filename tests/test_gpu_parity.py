@@ -209,11 +209,14 @@ def test_two_phase_fbm_surface_tension_step():
     vf = o.get(0, "volume_frac").astype(np.float64) + rng.uniform(-0.02, 0.02, (n, 2))
     vf = np.clip(vf, 0, None)
     vf = (vf / vf.sum(1, keepdims=True)).astype(np.float32)
-    o.set(0, "volume_frac", vf)
-    sim.fluid.from_numpy("volume_frac", vf)
-    sim.fluid.update_mass_all()
-    rd = o.get(0, "rest_density")
-    assert np.array_equal(sim.fluid.to_numpy("rest_density"), rd)
+    # rest_density / mass as push_part_seq forms them (sph_obj.py:178-180), f32
+    rho = [np.float32(1000), np.float32(500)]
+    rd = (rho[0] * vf[:, 0] + rho[1] * vf[:, 1]).astype(np.float32)
+    m = (rd * o.get(0, "rest_volume")).astype(np.float32)
+    for name, a in (("volume_frac", vf), ("rest_density", rd), ("mass", m)):
+        o.set(0, name, a)
+        sim.fluid.from_numpy(name, a)
+    assert np.array_equal(sim.fluid.to_numpy("rest_density"), o.get(0, "rest_density"))
     s = sim.sph_step(stats=True)
     st = o.step(O.STEP_CFL | O.STEP_FBM)
     assert (s.div_iter, s.incom_iter) == (st.div_iter, st.incom_iter)

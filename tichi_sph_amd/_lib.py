@@ -85,7 +85,7 @@ KERNEL_IDS = {
 EXPORTED = [
     "tsph_last_error", "tsph_abi_version", "tsph_control_bytes", "tsph_workspace_bytes",
     "tsph_sort_object", "tsph_ngrid_build", "tsph_neighbor_lists", "tsph_density_alpha",
-    "tsph_solver_begin", "tsph_solver_iteration", "tsph_advect", "tsph_fbm_round", "tsph_tail",
+    "tsph_solver_begin", "tsph_solver_iteration", "tsph_solver_psi", "tsph_solver_update", "tsph_advect", "tsph_fbm_round", "tsph_tail",
     "tsph_ref_kernel",
 ]
 
@@ -124,6 +124,8 @@ def load(path=None):
         "tsph_density_alpha": [P, O, O, C, _vp, sz, _vp],
         "tsph_solver_begin": [P, C, _i32, _vp],
         "tsph_solver_iteration": [P, O, O, C, _i32, _vp, sz, _vp],
+        "tsph_solver_psi": [P, O, O, C, _i32, _vp, sz, _vp],
+        "tsph_solver_update": [P, O, O, C, _i32, _vp, sz, _vp],
         "tsph_advect": [P, O, O, C, _vp, sz, _vp],
         "tsph_fbm_round": [P, O, C, _vp, sz, _vp],
         "tsph_tail": [P, O, C, _i32, _vp, sz, _vp],

@@ -27,8 +27,8 @@ int density_alpha(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, voi
 template <class T>
 int solver_begin(const TsphParams&, TsphControl*, int, hipStream_t);
 template <class T>
-int solver_iteration(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, void*, size_t,
-                     hipStream_t);
+int solver_half(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, int, void*, size_t,
+                hipStream_t);
 template <class T>
 int advect(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template <class T>
@@ -166,13 +166,27 @@ int tsph_solver_begin(const TsphParams* p, TsphControl* ctrl, int32_t solve, voi
   return DISPATCH(p, solver_begin, *p, ctrl, solve, (hipStream_t)stream);
 }
 
-int tsph_solver_iteration(const TsphParams* p, TsphObject* f, TsphObject* b, TsphControl* ctrl,
-                          int32_t k, void* ws, size_t wsb, void* stream) {
+static int solver_entry(const TsphParams* p, TsphObject* f, TsphObject* b, TsphControl* ctrl,
+                        int32_t k, int half, void* ws, size_t wsb, void* stream) {
   if (int rc = check_params(p)) return rc;
   if (int rc = check_obj(f, "tsph_solver_iteration")) return rc;
+  if (int rc = check_obj(b, "tsph_solver_iteration")) return rc;
   TSPH_REQUIRE(k >= 0 && k < TSPH_MAX_ITER_LOG, TSPH_EINVAL,
                "tsph_solver_iteration: iteration %d outside [0, %d)", k, TSPH_MAX_ITER_LOG);
-  return DISPATCH(p, solver_iteration, *p, f, b, ctrl, k, ws, wsb, (hipStream_t)stream);
+  return DISPATCH(p, solver_half, *p, f, b, ctrl, k, half, ws, wsb, (hipStream_t)stream);
+}
+
+int tsph_solver_iteration(const TsphParams* p, TsphObject* f, TsphObject* b, TsphControl* ctrl,
+                          int32_t k, void* ws, size_t wsb, void* stream) {
+  return solver_entry(p, f, b, ctrl, k, 3, ws, wsb, stream);
+}
+int tsph_solver_psi(const TsphParams* p, TsphObject* f, TsphObject* b, TsphControl* ctrl,
+                    int32_t k, void* ws, size_t wsb, void* stream) {
+  return solver_entry(p, f, b, ctrl, k, 1, ws, wsb, stream);
+}
+int tsph_solver_update(const TsphParams* p, TsphObject* f, TsphObject* b, TsphControl* ctrl,
+                       int32_t k, void* ws, size_t wsb, void* stream) {
+  return solver_entry(p, f, b, ctrl, k, 2, ws, wsb, stream);
 }
 
 int tsph_advect(const TsphParams* p, TsphObject* f, TsphObject* b, TsphControl* ctrl, void* ws,

@@ -243,26 +243,6 @@ __device__ __forceinline__ double block_min(double v, double* lds) {
   return s;
 }
 
-// Last-block election: every block calls this once after writing its
-// partial; returns true in exactly one block (the last to arrive), after
-// which that block sees every other block's partial.  The counter is reset
-// for the next launch by the elected block.
-__device__ __forceinline__ bool elect_last_block(int* counter, int nblocks, int* lds_flag) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    int prev = atomicAdd(counter, 1);
-    *lds_flag = (prev == nblocks - 1);
-  }
-  __syncthreads();
-  bool last = *lds_flag != 0;
-  if (last) {
-    __threadfence();
-    if (threadIdx.x == 0) *counter = 0;
-  }
-  return last;
-}
-
 // Deterministic fixed-order sum of `n` partials by one block (result in
 // thread 0).
 __device__ __forceinline__ double sum_partials(const double* part, int n, double* lds) {
@@ -274,6 +254,43 @@ __device__ __forceinline__ double min_partials(const double* part, int n, double
   double v = init;
   for (int i = threadIdx.x; i < n; i += blockDim.x) v = fmin(v, __builtin_nontemporal_load(part + i));
   return block_min(v, lds);
+}
+
+// Reductions are finished by a later kernel in stream order (kernel
+// boundaries order and publish the per-block partials); no block waits on
+// another inside a launch.
+enum FinalizeMode { kFinCfl = 0, kFinEnergy = 1, kFinFlag = 2, kFinCompression = 3 };
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_finalize(KP<T> k, int mode, const double* __restrict__ partial, int nblk, int n,
+               TsphControl* ctrl) {
+  __shared__ double lds[kWaves];
+  if (mode == kFinCfl) {
+    double all = min_partials(partial, nblk, lds, 1e300);
+    if (threadIdx.x == 0) {
+      TSPH_STRICT
+      T d = k.gps / (T)k.cs;  // sph.py:43
+      if (all < (double)d) d = (T)all;
+      ctrl->dt = (double)d;
+    }
+  } else if (mode == kFinEnergy) {
+    double a = sum_partials(partial, nblk, lds);
+    double b = sum_partials(partial + nblk, nblk, lds);
+    if (threadIdx.x == 0) {
+      ctrl->ke = a;
+      ctrl->pe = b;
+    }
+  } else if (mode == kFinFlag) {
+    double a = sum_partials(partial, nblk, lds);
+    if (threadIdx.x == 0) ctrl->general_flag = a > 0 ? 1 : 0;
+  } else {
+    double a = sum_partials(partial, nblk, lds);
+    if (threadIdx.x == 0) {
+      TSPH_STRICT
+      ctrl->compression = (double)((T)a / (T)n);  // sph.py:226
+    }
+  }
 }
 
 }  // namespace tsph

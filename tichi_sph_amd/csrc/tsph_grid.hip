@@ -102,19 +102,37 @@ __global__ void __launch_bounds__(kBlock)
            int* __restrict__ key, int* __restrict__ rank, int* __restrict__ count, int cfl,
            double* __restrict__ partial, TsphControl* ctrl) {
   __shared__ double lds[kWaves];
-  __shared__ int flag;
   int nb = gridDim.x;
   int b = xcd_block(blockIdx.x, nb);
   int i = b * kBlock + threadIdx.x;
   double cand = 1e300;
+  const int lane = threadIdx.x & 63;
+  int kk = -1 - lane;  // inactive lanes: unique keys that never merge
+  V4<T> p{};
   if (i < n) {
-    V4<T> p = pos[i];
+    p = pos[i];
     int c[3];
     node_encode(k, p.x, p.y, p.z, c);
     int code = dim_encode(k, c[0], c[1], c[2]);
-    int kk = (0 < code && code < k.node_num) ? code : k.node_num;  // invalid -> trailing bucket
+    kk = (0 < code && code < k.node_num) ? code : k.node_num;  // invalid -> trailing bucket
     key[i] = kk;
-    rank[i] = atomicAdd(&count[kk], 1);
+  }
+  // Arrival ranks: storage is in last step's cell order, so equal keys sit
+  // in adjacent lanes; each run of equal keys takes one atomic for its length.
+  {
+    int left = __shfl_up(kk, 1, 64);
+    bool lead = lane == 0 || left != kk;
+    unsigned long long leaders = __ballot(lead);
+    unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+    int start = 63 - __clzll(leaders & upto);
+    unsigned long long after = start == 63 ? 0ull : (leaders & ~((2ull << start) - 1));
+    int end = after ? __ffsll((long long)after) - 1 : 64;
+    int base = 0;
+    if (lead && i < n) base = atomicAdd(&count[kk], end - start);
+    base = __shfl(base, start, 64);
+    if (i < n) rank[i] = base + (lane - start);
+  }
+  if (i < n) {
     if (cfl) {  // cfl_condition, sph.py:44-47
       TSPH_STRICT
       V4<T> v = vel[i];
@@ -128,16 +146,6 @@ __global__ void __launch_bounds__(kBlock)
   if (!cfl) return;
   double m = block_min(cand, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = m;
-  if (elect_last_block(&ctrl->counter, nb, &flag)) {
-    double all = min_partials(partial, nb, lds, 1e300);
-    if (threadIdx.x == 0) {
-      TSPH_STRICT
-      T dt0 = k.gps / (T)k.cs;  // sph.py:43
-      T d = dt0;
-      if (all < (double)dt0) d = (T)all;
-      ctrl->dt = (double)d;
-    }
-  }
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -199,6 +207,9 @@ int sort_object(const TsphParams& prm, TsphObject* o, TsphControl* ctrl, int cfl
     hipLaunchKernelGGL(k_keys<T>, dim3(nbl), dim3(kBlock), 0, st, k, (const V4<T>*)o->pos,
                        (const V4<T>*)o->vel, n, L.key, L.rank, o->cell_count, cfl, L.partial,
                        ctrl);
+    if (cfl)
+      hipLaunchKernelGGL(k_finalize<T>, dim3(1), dim3(kBlock), 0, st, k, (int)kFinCfl, L.partial,
+                         nbl, n, ctrl);
   }
   TSPH_LAUNCH_CHECK("keys");
   if (int rc = launch_scan(o->cell_count, len, L.scan_sums, o->cell_start, st)) return rc;

@@ -100,7 +100,6 @@ __global__ void __launch_bounds__(kBlock)
           TsphControl* ctrl) {
   TSPH_STRICT
   __shared__ double lds[kWaves];
-  __shared__ int lflag;
   const int nblk = gridDim.x;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   const bool act = i < o.n;
@@ -391,34 +390,13 @@ __global__ void __launch_bounds__(kBlock)
     default:
       return;
   }
-  // reductions: block partials, last block finalises
+  // reductions: block partials, finished by k_finalize in stream order
   double s0 = block_sum(red0, lds);
   double s1 = block_sum(red1, lds);
   double mn = block_min(redmin, lds);
   if (threadIdx.x == 0) {
     partial[blockIdx.x] = kid == TSPH_K_cfl_condition ? mn : s0;
     partial[nblk + blockIdx.x] = s1;
-  }
-  if (elect_last_block(&ctrl->counter, nblk, &lflag)) {
-    if (kid == TSPH_K_cfl_condition) {
-      double all = min_partials(partial, nblk, lds, 1e300);
-      if (threadIdx.x == 0) {
-        T d = k.gps / (T)k.cs;
-        if (all < (double)d) d = (T)all;
-        ctrl->dt = (double)d;
-      }
-    } else {
-      double a = sum_partials(partial, nblk, lds);
-      double b = sum_partials(partial + nblk, nblk, lds);
-      if (threadIdx.x == 0) {
-        if (kid == TSPH_K_IPPE_psi_adv_non_negative) ctrl->compression = (double)((T)a / (T)o.n);
-        if (kid == TSPH_K_SPH_update_energy) {
-          ctrl->ke = a;
-          ctrl->pe = b;
-        }
-        if (kid == TSPH_K_SPH_FBM_check_tmp) ctrl->general_flag = a > 0 ? 1 : 0;
-      }
-    }
   }
 }
 
@@ -554,6 +532,13 @@ int ref_kernel(const TsphParams& prm, int kid, TsphObject* obj, const TsphObject
   } else {
     hipLaunchKernelGGL(k_ref<T>, dim3(nb), dim3(kBlock), 0, st, kid, k, g, o, no, L.partial,
                        ctrl);
+    int mode = kid == TSPH_K_cfl_condition ? kFinCfl
+               : kid == TSPH_K_SPH_update_energy ? kFinEnergy
+               : kid == TSPH_K_SPH_FBM_check_tmp ? kFinFlag
+               : kid == TSPH_K_IPPE_psi_adv_non_negative ? kFinCompression : -1;
+    if (mode >= 0)
+      hipLaunchKernelGGL(k_finalize<T>, dim3(1), dim3(kBlock), 0, st, k, mode, L.partial, nb,
+                         obj->n, ctrl);
   }
   TSPH_LAUNCH_CHECK("ref_kernel");
   return 0;

@@ -39,11 +39,71 @@ __device__ __forceinline__ T rest_psi_of(const KP<T>& k, const T* rdens, int i) 
   return k.solver == TSPH_VFSPH ? T(1) : rdens[i];
 }
 
+// Pair terms of pass A for one in-range candidate (strict arithmetic, the
+// order of sph.py:61-64, 81, 97).
+template <class T>
+struct PassAAcc {
+  T w = 0, cp = 0, dn = 0, ax = 0, ay = 0, az = 0, a2 = 0;
+  __device__ __forceinline__ bool add(const KP<T>& k, const V4<T>& p, const V4<T>& q, T mj, T vj,
+                                      bool fluid) {
+    TSPH_STRICT
+    T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    T r = s_sqrt(s_r2(dx, dy, dz));
+    T Wr = s_W(k, r);
+    w = w + Wr;
+    cp = cp + Wr * vj;
+    dn = dn + Wr * mj;
+    if (!(r > T(0))) return false;
+    T wg = s_Wg(k, r);
+    T Xj = q.w;
+    ax = ax + Xj * dx / r * wg;
+    ay = ay + Xj * dy / r * wg;
+    az = az + Xj * dz / r * wg;
+    if (fluid) {
+      T wg2 = wg * wg;
+      a2 = a2 + wg2 * (Xj * Xj) / mj;
+    }
+    return true;
+  }
+};
+
 // ---------------------------------------------------------------------------
 // Pass A, fluid particles: SPH_prepare_attr (f,f)(f,b), SPH_prepare_alpha_1
 // (f,f)(f,b), SPH_prepare_alpha_2 (f,f), SPH_prepare_alpha, SPH_vel_2_vel_adv,
 // neighbour lists.  Strict arithmetic: bit-for-bit the reference's order.
 // ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ void pass_a_epilogue(const KP<T>& k, int i, const V4<T>& p, T mi,
+                                                const PassAAcc<T>& a, int nf, int nbb,
+                                                T* W, T* comp, T* dens, V4<T>* alpha_1,
+                                                T* alpha_2, T* alpha, T* vol, const V4<T>* vel,
+                                                V4<T>* vel_adv, int* nbr_nf, int* nbr_nb,
+                                                TsphControl* ctrl) {
+  TSPH_STRICT
+  W[i] = a.w;
+  comp[i] = a.cp;
+  dens[i] = a.dn;
+  alpha_1[i] = V4<T>{a.ax, a.ay, a.az, T(0)};
+  alpha_2[i] = a.a2;
+  T dot = s_r2(a.ax, a.ay, a.az);
+  T al = dot / mi + a.a2;  // SPH_prepare_alpha, sph.py:103-105
+  if (al < T(1e-4)) al = T(1e-4);
+  alpha[i] = al;
+  T psi = k.solver == TSPH_VFSPH ? a.cp : a.dn;
+  vol[i] = p.w / psi;   // X / sph_psi, reused by the viscosity and normal sweeps
+  vel_adv[i] = vel[i];  // SPH_vel_2_vel_adv, sph.py:253-256
+  if (nf > k.list_cap_f) {
+    atomicAdd(&ctrl->overflow_f, 1);
+    nf = k.list_cap_f + 1;
+  }
+  if (nbb > k.list_cap_b) {
+    atomicAdd(&ctrl->overflow_b, 1);
+    nbb = k.list_cap_b + 1;
+  }
+  nbr_nf[i] = nf;
+  nbr_nb[i] = nbb;
+}
+
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_pass_a_fluid(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
@@ -56,83 +116,34 @@ __global__ void __launch_bounds__(kBlock)
                    V4<T>* __restrict__ vel_adv, int* __restrict__ nbr_f, int* __restrict__ nbr_b,
                    int* __restrict__ nbr_nf, int* __restrict__ nbr_nb, int* __restrict__ flag,
                    TsphControl* ctrl) {
-  TSPH_STRICT
-  int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i >= n) return;
+  const int capf = k.list_cap_f, capb = k.list_cap_b;
   flag[i] = 0;  // SPH_clean_value, sph.py:30
-  V4<T> p = fpos[i];
-  T mi = fmass[i];
+  const V4<T> p = fpos[i];
+  const T mi = fmass[i];
   int c[3];
   node_encode(k, p.x, p.y, p.z, c);
-  T w = 0, cp = 0, dn = 0, ax = 0, ay = 0, az = 0, a2 = 0;
+  PassAAcc<T> acc;
   int nf = 0, nbb = 0;
-  const int capf = k.list_cap_f, capb = k.list_cap_b;
   walk_cells(k, fstart, c, [&](int j) {
-    V4<T> q = fpos[j];
-    T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-    T r2 = s_r2(dx, dy, dz);
-    if (r2 <= k.r2max) {
-      T r = s_sqrt(r2);
-      T Wr = s_W(k, r);
-      w = w + Wr;
-      cp = cp + Wr * frvol[j];
-      dn = dn + Wr * fmass[j];
-      if (r > T(0)) {
-        T wg = s_Wg(k, r);
-        T Xj = q.w;
-        ax = ax + Xj * dx / r * wg;
-        ay = ay + Xj * dy / r * wg;
-        az = az + Xj * dz / r * wg;
-        T wg2 = wg * wg;
-        a2 = a2 + wg2 * (Xj * Xj) / fmass[j];
-        if (nf < capf) nbr_f[(long)nf * cap + i] = j;
-        ++nf;
-      }
+    const V4<T> q = fpos[j];
+    if (s_r2(p.x - q.x, p.y - q.y, p.z - q.z) <= k.r2max &&
+        acc.add(k, p, q, fmass[j], frvol[j], true)) {
+      if (nf < capf) nbr_f[(long)nf * cap + i] = j;
+      ++nf;
     }
   });
   walk_cells(k, bstart, c, [&](int j) {
-    V4<T> q = bpos[j];
-    T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-    T r2 = s_r2(dx, dy, dz);
-    if (r2 <= k.r2max) {
-      T r = s_sqrt(r2);
-      T Wr = s_W(k, r);
-      w = w + Wr;
-      cp = cp + Wr * brvol[j];
-      dn = dn + Wr * bmass[j];
-      if (r > T(0)) {
-        T wg = s_Wg(k, r);
-        T Xj = q.w;
-        ax = ax + Xj * dx / r * wg;
-        ay = ay + Xj * dy / r * wg;
-        az = az + Xj * dz / r * wg;
-        if (nbb < capb) nbr_b[(long)nbb * cap + i] = j;
-        ++nbb;
-      }
+    const V4<T> q = bpos[j];
+    if (s_r2(p.x - q.x, p.y - q.y, p.z - q.z) <= k.r2max &&
+        acc.add(k, p, q, bmass[j], brvol[j], false)) {
+      if (nbb < capb) nbr_b[(long)nbb * cap + i] = j;
+      ++nbb;
     }
   });
-  W[i] = w;
-  comp[i] = cp;
-  dens[i] = dn;
-  alpha_1[i] = V4<T>{ax, ay, az, T(0)};
-  alpha_2[i] = a2;
-  T dot = s_r2(ax, ay, az);
-  T al = dot / mi + a2;  // SPH_prepare_alpha, sph.py:103-105
-  if (al < T(1e-4)) al = T(1e-4);
-  alpha[i] = al;
-  T psi = k.solver == TSPH_VFSPH ? cp : dn;
-  vol[i] = p.w / psi;  // X / sph_psi, reused by the viscosity and normal sweeps
-  vel_adv[i] = vel[i];  // SPH_vel_2_vel_adv, sph.py:253-256
-  if (nf > capf) {
-    atomicAdd(&ctrl->overflow_f, 1);
-    nf = capf + 1;
-  }
-  if (nbb > capb) {
-    atomicAdd(&ctrl->overflow_b, 1);
-    nbb = capb + 1;
-  }
-  nbr_nf[i] = nf;
-  nbr_nb[i] = nbb;
+  pass_a_epilogue(k, i, p, mi, acc, nf, nbb, W, comp, dens, alpha_1, alpha_2, alpha, vol, vel,
+                  vel_adv, nbr_nf, nbr_nb, ctrl);
 }
 
 // Pass A, boundary particles: SPH_prepare_attr (b,b)(b,f), SPH_prepare_alpha_2
@@ -209,13 +220,14 @@ __global__ void k_step_reset(TsphControl* ctrl) {
 template <class T>
 __global__ void k_solver_begin(KP<T> k, TsphControl* ctrl, int solve) {
   int t = threadIdx.x;
-  for (int q = t; q < TSPH_MAX_ITER_LOG; q += blockDim.x) ctrl->run_u[q] = 0;
+  T thr = solve == 0 ? k.div_thr : k.comp_thr;
+  T comp = (T)ctrl->compression;
+  bool enter = (0 < k.iter_min) || (comp > thr);
+  // run_u[q] = 1 iff iteration q of this solve runs; decided one iteration ahead
+  for (int q = t; q < TSPH_MAX_ITER_LOG; q += blockDim.x) ctrl->run_u[q] = (q == 0 && enter) ? 1 : 0;
   if (t == 0) {
     ctrl->it = 0;
     ctrl->solve = solve;
-    T thr = solve == 0 ? k.div_thr : k.comp_thr;
-    T comp = (T)ctrl->compression;
-    bool enter = (0 < k.iter_min) || (comp > thr);
     ctrl->stop = enter ? 0 : 1;
     if (solve == 0) ctrl->div_iter = 0;
     else ctrl->incom_iter = 0;
@@ -224,9 +236,9 @@ __global__ void k_solver_begin(KP<T> k, TsphControl* ctrl, int solve) {
 
 // Iterate a particle's fluid neighbours: the recorded list, or, when the list
 // overflowed, the cell walk with the same in-range test.
-template <class T, class F>
+template <class T, class L, class F>
 __device__ __forceinline__ void for_fluid_nbrs(const KP<T>& k, int i, int cap, int nf,
-                                               const int* __restrict__ list,
+                                               const L& list,
                                                const int* __restrict__ cstart,
                                                const V4<T>* __restrict__ pos, V4<T> p, F&& f) {
   if (nf <= k.list_cap_f) {
@@ -259,6 +271,70 @@ __device__ __forceinline__ void for_bound_nbrs(const KP<T>& k, int i, int cap, i
   }
 }
 
+// Neighbour iteration of the list sweeps for one lane, in the recorded
+// (reference) order: fluid list, then boundary list.  The sweeps are bound
+// by the latency of dependent loads (list entry -> neighbour data), so the
+// lists are consumed in batches of kNb: all kNb indices are loaded, then all
+// their neighbour records, then the pair terms are applied in order.  A lane
+// whose list overflowed walks the cells with the same in-range test.
+constexpr int kNb = 8;
+
+template <class T, class Rec, class LoadF, class LoadB, class PairF, class PairB>
+__device__ __forceinline__ void sweep_lists(const KP<T>& k, int cap, int i, V4<T> p,
+                                            const int* __restrict__ fstart,
+                                            const int* __restrict__ bstart,
+                                            const V4<T>* __restrict__ fpos,
+                                            const V4<T>* __restrict__ bpos,
+                                            const int* __restrict__ nbr_f,
+                                            const int* __restrict__ nbr_b, int nf, int nbb,
+                                            LoadF&& load_f, LoadB&& load_b, PairF&& pair_f,
+                                            PairB&& pair_b) {
+  if (nf <= k.list_cap_f && nbb <= k.list_cap_b) {
+    for (int s0 = 0; s0 < nf; s0 += kNb) {
+      int j[kNb];
+      Rec d[kNb];
+#pragma unroll
+      for (int u = 0; u < kNb; ++u) j[u] = s0 + u < nf ? nbr_f[(long)(s0 + u) * cap + i] : -1;
+#pragma unroll
+      for (int u = 0; u < kNb; ++u)
+        if (j[u] >= 0) d[u] = load_f(j[u]);
+#pragma unroll
+      for (int u = 0; u < kNb; ++u)
+        if (j[u] >= 0) pair_f(d[u]);
+    }
+    for (int s0 = 0; s0 < nbb; s0 += kNb) {
+      int j[kNb];
+      Rec d[kNb];
+#pragma unroll
+      for (int u = 0; u < kNb; ++u) j[u] = s0 + u < nbb ? nbr_b[(long)(s0 + u) * cap + i] : -1;
+#pragma unroll
+      for (int u = 0; u < kNb; ++u)
+        if (j[u] >= 0) d[u] = load_b(j[u]);
+#pragma unroll
+      for (int u = 0; u < kNb; ++u)
+        if (j[u] >= 0) pair_b(d[u]);
+    }
+    return;
+  }
+  int c[3];
+  node_encode(k, p.x, p.y, p.z, c);
+  walk_cells(k, fstart, c, [&](int j) {
+    V4<T> q = fpos[j];
+    T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
+    if (r2 > T(0) && r2 <= k.r2max) pair_f(load_f(j));
+  });
+  walk_cells(k, bstart, c, [&](int j) {
+    V4<T> q = bpos[j];
+    T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
+    if (r2 > T(0) && r2 <= k.r2max) pair_b(load_b(j));
+  });
+}
+
+template <class T>
+struct Rec2 {
+  V4<T> a, b;
+};
+
 // Jacobi half-step P: IPPE_adv_psi_init + IPPE_adv_psi (f,f)(f,b) +
 // IPPE_psi_adv_non_negative (sph.py:196-226) for iteration kk of the solve.
 template <class T>
@@ -272,38 +348,29 @@ __global__ void __launch_bounds__(kBlock)
                  const int* __restrict__ nbr_nb, T* __restrict__ psi_adv, T* __restrict__ kappa,
                  double* __restrict__ partial, TsphControl* ctrl) {
   __shared__ double lds[kWaves];
-  __shared__ int flag;
-  if (ctrl->stop) return;
-  int nblk = gridDim.x;
-  int i = xcd_block(blockIdx.x, nblk) * kBlock + threadIdx.x;
-  const T dt = (T)ctrl->dt;
+  if (!ctrl->run_u[kk]) return;
+  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   double err = 0;
   if (i < n) {
-    V4<T> p = fpos[i];
-    V4<T> va = vadv[i];
-    T rp = rest_psi_of(k, rdens, i);
-    T psi;
-    {
-      TSPH_STRICT
-      psi = psi_rho[i] - rp;
-    }
-    for_fluid_nbrs(k, i, cap, nbr_nf[i], nbr_f, fstart, fpos, p, [&](int j) {
-      V4<T> q = fpos[j];
-      V4<T> vj = vadv[j];
+    const T dt = (T)ctrl->dt;
+    const V4<T> p = fpos[i];
+    const V4<T> va = vadv[i];
+    const T rp = rest_psi_of(k, rdens, i);
+    T psi = psi_rho[i] - rp;  // a single rounding
+    auto term = [&](const Rec2<T>& d) {
+      const V4<T>& q = d.a;
+      const V4<T>& vj = d.b;
       T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
       T r2 = dx * dx + dy * dy + dz * dz, r;
       T g = f_grad(k, r2, &r);
       T dot = (dx * (va.x - vj.x) + dy * (va.y - vj.y) + dz * (va.z - vj.z)) * g;
       psi += dot * q.w * dt;
-    });
-    for_bound_nbrs(k, i, cap, nbr_nb[i], nbr_b, bstart, bpos, p, [&](int j) {
-      V4<T> q = bpos[j];
-      T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-      T r2 = dx * dx + dy * dy + dz * dz, r;
-      T g = f_grad(k, r2, &r);
-      T dot = (dx * va.x + dy * va.y + dz * va.z) * g;  // boundary vel_adv is 0
-      psi += dot * q.w * dt;
-    });
+    };
+    sweep_lists<T, Rec2<T>>(
+        k, cap, i, p, fstart, bstart, fpos, bpos, nbr_f, nbr_b, nbr_nf[i], nbr_nb[i],
+        [&](int j) { return Rec2<T>{fpos[j], vadv[j]}; },
+        // boundary vel_adv is never written: 0
+        [&](int j) { return Rec2<T>{bpos[j], V4<T>{T(0), T(0), T(0), T(0)}}; }, term, term);
     if (psi < T(0)) psi = T(0);
     psi_adv[i] = psi;
     kappa[i] = psi / alpha[i];
@@ -311,31 +378,11 @@ __global__ void __launch_bounds__(kBlock)
   }
   double s = block_sum(err, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
-  if (elect_last_block(&ctrl->counter, nblk, &flag)) {
-    double tot = sum_partials(partial, nblk, lds);
-    if (threadIdx.x == 0) {
-      TSPH_STRICT
-      T comp = (T)tot / (T)n;  // sph.py:226
-      ctrl->compression = (double)comp;
-      int solve = ctrl->solve;
-      if (kk < TSPH_MAX_ITER_LOG) {
-        if (solve == 0) ctrl->err_div[kk] = (double)comp;
-        else ctrl->err_incom[kk] = (double)comp;
-        ctrl->run_u[kk] = 1;
-      }
-      int it = kk + 1;
-      ctrl->it = it;
-      if (solve == 0) ctrl->div_iter = it;
-      else ctrl->incom_iter = it;
-      T thr = solve == 0 ? k.div_thr : k.comp_thr;
-      bool cont = (it < k.iter_min) || (comp > thr);
-      ctrl->stop = (it > k.iter_max || !cont) ? 1 : 0;
-    }
-  }
 }
 
 // Jacobi half-step U: IPPE_update_vel_adv (f,f)(f,b) (sph.py:229-244), with
-// kappa = psi_adv / alpha formed once per particle.
+// kappa = psi_adv / alpha formed once per particle.  Block 0 also finishes
+// iteration kk's compression and takes the loop decision for iteration kk+1.
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_solver_update(KP<T> k, int n, int cap, int kk, const V4<T>* __restrict__ fpos,
@@ -344,37 +391,62 @@ __global__ void __launch_bounds__(kBlock)
                     const int* __restrict__ bstart, const int* __restrict__ nbr_f,
                     const int* __restrict__ nbr_b, const int* __restrict__ nbr_nf,
                     const int* __restrict__ nbr_nb, V4<T>* __restrict__ vadv,
-                    TsphControl* ctrl) {
-  if (kk >= TSPH_MAX_ITER_LOG || !ctrl->run_u[kk]) return;
-  int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+                    const double* __restrict__ partial, TsphControl* ctrl) {
+  __shared__ double lds[kWaves];
+  if (!ctrl->run_u[kk]) return;
+  if (blockIdx.x == 0) {
+    // IPPE_psi_adv_non_negative's 0-D reduction (sph.py:221-226) and the loop
+    // test of main.py:201-213 / 222-234
+    double tot = sum_partials(partial, gridDim.x, lds);
+    if (threadIdx.x == 0) {
+      TSPH_STRICT
+      T comp = (T)tot / (T)n;
+      ctrl->compression = (double)comp;
+      int solve = ctrl->solve;
+      if (solve == 0) ctrl->err_div[kk] = (double)comp;
+      else ctrl->err_incom[kk] = (double)comp;
+      int it = kk + 1;
+      ctrl->it = it;
+      if (solve == 0) ctrl->div_iter = it;
+      else ctrl->incom_iter = it;
+      T thr = solve == 0 ? k.div_thr : k.comp_thr;
+      bool cont = (it < k.iter_min) || (comp > thr);
+      bool run = it <= k.iter_max && cont && it < TSPH_MAX_ITER_LOG;
+      if (it < TSPH_MAX_ITER_LOG) ctrl->run_u[it] = run ? 1 : 0;
+      ctrl->stop = run ? 0 : 1;
+    }
+  }
+  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i >= n) return;
   const T dt = (T)ctrl->dt;
-  V4<T> p = fpos[i];
+  const V4<T> p = fpos[i];
   V4<T> va = vadv[i];
-  T ki = kappa[i];
-  T Xi = p.w;
-  T c = -(T(1) / dt) / fmass[i];
-  for_fluid_nbrs(k, i, cap, nbr_nf[i], nbr_f, fstart, fpos, p, [&](int j) {
-    V4<T> q = fpos[j];
-    T kj = kappa[j];
+  const T ki = kappa[i];
+  const T Xi = p.w;
+  const T c = -(T(1) / dt) / fmass[i];
+  auto term = [&](const V4<T>& q) {  // q.w carries kappa_j * X_i + kappa_i * X_j
     T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     T r2 = dx * dx + dy * dy + dz * dz, r;
     T g = f_grad(k, r2, &r);
-    T s = c * (ki * q.w + kj * Xi) * g;
+    T s = c * q.w * g;
     va.x += s * dx;
     va.y += s * dy;
     va.z += s * dz;
-  });
-  for_bound_nbrs(k, i, cap, nbr_nb[i], nbr_b, bstart, bpos, p, [&](int j) {
-    V4<T> q = bpos[j];
-    T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-    T r2 = dx * dx + dy * dy + dz * dz, r;
-    T g = f_grad(k, r2, &r);
-    T s = c * (ki * q.w) * g;  // boundary psi_adv is 0
-    va.x += s * dx;
-    va.y += s * dy;
-    va.z += s * dz;
-  });
+  };
+  sweep_lists<T, V4<T>>(
+      k, cap, i, p, fstart, bstart, fpos, bpos, nbr_f, nbr_b, nbr_nf[i], nbr_nb[i],
+      [&](int j) {
+        V4<T> q = fpos[j];
+        q.w = ki * q.w + kappa[j] * Xi;
+        return q;
+      },
+      // boundary psi_adv is never written (0), so only the kappa_i term remains
+      [&](int j) {
+        V4<T> q = bpos[j];
+        q.w = ki * q.w;
+        return q;
+      },
+      term, term);
   vadv[i] = va;
 }
 
@@ -386,23 +458,23 @@ __global__ void __launch_bounds__(kBlock)
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ void __launch_bounds__(kBlock)
-    k_advect(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
-             const T* __restrict__ vol, const T* __restrict__ rdens,
-             const int* __restrict__ fstart, const int* __restrict__ nbr_f,
-             const int* __restrict__ nbr_nf, V4<T>* __restrict__ vadv, V4<T>* __restrict__ vel,
-             V4<T>* __restrict__ acce_adv, V4<T>* __restrict__ normal) {
-  int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    k_advect(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos, const T* __restrict__ vol,
+             const T* __restrict__ rdens, const int* __restrict__ fstart,
+             const int* __restrict__ bstart, const V4<T>* __restrict__ bpos,
+             const int* __restrict__ nbr_f, const int* __restrict__ nbr_nf,
+             V4<T>* __restrict__ vadv, V4<T>* __restrict__ vel, V4<T>* __restrict__ acce_adv,
+             V4<T>* __restrict__ normal) {
+  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i >= n) return;
-  V4<T> p = fpos[i];
-  V4<T> vi = vadv[i];
-  vel[i] = vi;  // SPH_vel_adv_2_vel
+  const V4<T> p = fpos[i];
+  const V4<T> vi = vadv[i];
+  const T fvis = k.two_d2 * k.mu / rdens[i];
   T ax = T(0) + k.g[0], ay = T(0) + k.g[1], az = T(0) + k.g[2];
   T nx = 0, ny = 0, nz = 0;
-  const T fvis = k.two_d2 * k.mu / rdens[i];
-  for_fluid_nbrs(k, i, cap, nbr_nf[i], nbr_f, fstart, fpos, p, [&](int j) {
-    V4<T> q = fpos[j];
-    V4<T> vj = vadv[j];
-    T Vj = vol[j];
+  auto term = [&](const Rec2<T>& d) {
+    const V4<T>& q = d.a;
+    const V4<T>& vj = d.b;
+    T Vj = vj.w;  // vol = X / sph_psi rides in the velocity's 4th lane
     T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     T r2 = dx * dx + dy * dy + dz * dz, r;
     T g = f_grad(k, r2, &r);
@@ -415,7 +487,17 @@ __global__ void __launch_bounds__(kBlock)
     nx += t * dx;
     ny += t * dy;
     nz += t * dz;
-  });
+  };
+  // viscosity and normals are (f,f) only: the boundary list is not visited
+  sweep_lists<T, Rec2<T>>(
+      k, cap, i, p, fstart, bstart, fpos, bpos, nbr_f, nullptr, nbr_nf[i], 0,
+      [&](int j) {
+        V4<T> v = vadv[j];
+        v.w = vol[j];
+        return Rec2<T>{fpos[j], v};
+      },
+      [&](int j) { return Rec2<T>{}; }, term, [&](const Rec2<T>&) {});
+  vel[i] = vi;  // SPH_vel_adv_2_vel
   acce_adv[i] = V4<T>{ax, ay, az, T(0)};
   normal[i] = V4<T>{nx * k.h, ny * k.h, nz * k.h, T(0)};
 }
@@ -512,12 +594,13 @@ __global__ void __launch_bounds__(kBlock)
     k_fbm_flux(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
                const T* __restrict__ rvol, const T* __restrict__ vf,
                const V4<T>* __restrict__ drift, const int* __restrict__ flag,
-               const int* __restrict__ fstart, const int* __restrict__ nbr_f,
+               const int* __restrict__ fstart, const int* __restrict__ nbr_f_in,
                const int* __restrict__ nbr_nf, T* __restrict__ tmp, const TsphControl* ctrl) {
   TSPH_STRICT
   if (!ctrl->general_flag) return;
   int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i >= n || flag[i] != 0) return;
+  const int* nbr_f = nbr_f_in;
   const T dt = (T)ctrl->dt;
   const int P = k.P;
   V4<T> p = fpos[i];
@@ -571,7 +654,6 @@ __global__ void __launch_bounds__(kBlock)
                 int* __restrict__ flag, double* __restrict__ partial, TsphControl* ctrl) {
   TSPH_STRICT
   __shared__ double lds[kWaves];
-  __shared__ int lflag;
   if (!ctrl->general_flag) return;
   int nblk = gridDim.x;
   int i = xcd_block(blockIdx.x, nblk) * kBlock + threadIdx.x;
@@ -587,10 +669,16 @@ __global__ void __launch_bounds__(kBlock)
   }
   double s = block_sum(any, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
-  if (elect_last_block(&ctrl->counter, nblk, &lflag)) {
-    double tot = sum_partials(partial, nblk, lds);
-    if (threadIdx.x == 0) ctrl->general_flag = tot > 0 ? 1 : 0;
-  }
+}
+
+// general_flag = any particle flagged this round (sph.py:368-372); skipped,
+// like the round itself, once the flag is clear.
+__global__ void __launch_bounds__(kBlock)
+    k_fbm_check_finish(const double* __restrict__ partial, int nblk, TsphControl* ctrl) {
+  __shared__ double lds[kWaves];
+  if (!ctrl->general_flag) return;
+  double a = sum_partials(partial, nblk, lds);
+  if (threadIdx.x == 0) ctrl->general_flag = a > 0 ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -605,7 +693,6 @@ __global__ void __launch_bounds__(kBlock)
            const int* __restrict__ flag, double* __restrict__ partial, TsphControl* ctrl) {
   TSPH_STRICT
   __shared__ double lds[kWaves];
-  __shared__ int lflag;
   int nblk = gridDim.x;
   int i = xcd_block(blockIdx.x, nblk) * kBlock + threadIdx.x;
   const T dt = (T)ctrl->dt;
@@ -637,14 +724,6 @@ __global__ void __launch_bounds__(kBlock)
     partial[blockIdx.x] = s1;
     partial[nblk + blockIdx.x] = s2;
   }
-  if (elect_last_block(&ctrl->counter, nblk, &lflag)) {
-    double a = sum_partials(partial, nblk, lds);
-    double b = sum_partials(partial + nblk, nblk, lds);
-    if (threadIdx.x == 0) {
-      ctrl->ke = a;
-      ctrl->pe = b;
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -654,6 +733,9 @@ template <class T>
 int density_alpha(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl* ctrl,
                   void* ws, size_t wsb, hipStream_t st) {
   KP<T> k = make_kp<T>(prm);
+  WsLayout L;
+  TSPH_REQUIRE(carve_ws(ws, wsb, f->capacity, prm.node_num, &L), TSPH_ECAPACITY,
+               "tsph_density_alpha: workspace too small");
   hipLaunchKernelGGL(k_step_reset, dim3(1), dim3(64), 0, st, ctrl);
   if (f->n > 0)
     hipLaunchKernelGGL(k_pass_a_fluid<T>, dim3(blocks_for(f->n)), dim3(kBlock), 0, st, k, f->n,
@@ -662,7 +744,8 @@ int density_alpha(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphContr
                        (const T*)b->mass, (const T*)b->rest_volume, b->cell_start, (T*)f->W,
                        (T*)f->sph_compression, (T*)f->sph_density, (V4<T>*)f->alpha_1,
                        (T*)f->alpha_2, (T*)f->alpha, (T*)f->vol, (const V4<T>*)f->vel,
-                       (V4<T>*)f->vel_adv, f->nbr_f, f->nbr_b, f->nbr_nf, f->nbr_nb, f->flag, ctrl);
+                       (V4<T>*)f->vel_adv, f->nbr_f, f->nbr_b, f->nbr_nf, f->nbr_nb, f->flag,
+                       ctrl);
   if (b->n > 0)
     hipLaunchKernelGGL(k_pass_a_bound<T>, dim3(blocks_for(b->n)), dim3(kBlock), 0, st, k, b->n,
                        (const V4<T>*)b->pos, (const T*)b->mass, (const T*)b->rest_volume,
@@ -683,8 +766,8 @@ int solver_begin(const TsphParams& prm, TsphControl* ctrl, int solve, hipStream_
 }
 
 template <class T>
-int solver_iteration(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl* ctrl,
-                     int kk, void* ws, size_t wsb, hipStream_t st) {
+int solver_half(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl* ctrl, int kk,
+                int half, void* ws, size_t wsb, hipStream_t st) {
   KP<T> k = make_kp<T>(prm);
   WsLayout L;
   TSPH_REQUIRE(carve_ws(ws, wsb, f->capacity, prm.node_num, &L), TSPH_ECAPACITY,
@@ -692,15 +775,17 @@ int solver_iteration(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphCo
   TSPH_REQUIRE(f->n > 0, TSPH_EINVAL, "tsph_solver_iteration: no fluid particles");
   int nb = blocks_for(f->n);
   const T* psi = (const T*)(prm.solver_type == TSPH_VFSPH ? f->sph_compression : f->sph_density);
-  hipLaunchKernelGGL(k_solver_psi<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity, kk,
-                     (const V4<T>*)f->pos, (const V4<T>*)f->vel_adv, psi,
-                     (const T*)f->rest_density, (const T*)f->alpha, f->cell_start,
-                     (const V4<T>*)b->pos, b->cell_start, f->nbr_f, f->nbr_b, f->nbr_nf,
-                     f->nbr_nb, (T*)f->psi_adv, (T*)f->kappa, L.partial, ctrl);
-  hipLaunchKernelGGL(k_solver_update<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity, kk,
-                     (const V4<T>*)f->pos, (const T*)f->mass, (const T*)f->kappa, f->cell_start,
-                     (const V4<T>*)b->pos, b->cell_start, f->nbr_f, f->nbr_b, f->nbr_nf,
-                     f->nbr_nb, (V4<T>*)f->vel_adv, ctrl);
+  if (half & 1)
+    hipLaunchKernelGGL(k_solver_psi<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity, kk,
+                       (const V4<T>*)f->pos, (const V4<T>*)f->vel_adv, psi,
+                       (const T*)f->rest_density, (const T*)f->alpha, f->cell_start,
+                       (const V4<T>*)b->pos, b->cell_start, f->nbr_f, f->nbr_b, f->nbr_nf,
+                       f->nbr_nb, (T*)f->psi_adv, (T*)f->kappa, L.partial, ctrl);
+  if (half & 2)
+    hipLaunchKernelGGL(k_solver_update<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
+                       kk, (const V4<T>*)f->pos, (const T*)f->mass, (const T*)f->kappa,
+                       f->cell_start, (const V4<T>*)b->pos, b->cell_start, f->nbr_f, f->nbr_b,
+                       f->nbr_nf, f->nbr_nb, (V4<T>*)f->vel_adv, L.partial, ctrl);
   TSPH_LAUNCH_CHECK("solver_iteration");
   return 0;
 }
@@ -709,12 +794,16 @@ template <class T>
 int advect(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl* ctrl, void* ws,
            size_t wsb, hipStream_t st) {
   KP<T> k = make_kp<T>(prm);
+  WsLayout L;
+  TSPH_REQUIRE(carve_ws(ws, wsb, f->capacity, prm.node_num, &L), TSPH_ECAPACITY,
+               "tsph_advect: workspace too small");
   if (f->n == 0) return 0;
   int nb = blocks_for(f->n);
   hipLaunchKernelGGL(k_advect<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
                      (const V4<T>*)f->pos, (const T*)f->vol, (const T*)f->rest_density,
-                     f->cell_start, f->nbr_f, f->nbr_nf, (V4<T>*)f->vel_adv, (V4<T>*)f->vel,
-                     (V4<T>*)f->acce_adv, (V4<T>*)f->normal);
+                     f->cell_start, b->cell_start, (const V4<T>*)b->pos, f->nbr_f, f->nbr_nf,
+                     (V4<T>*)f->vel_adv, (V4<T>*)f->vel, (V4<T>*)f->acce_adv,
+                     (V4<T>*)f->normal);
   const T* psi = (const T*)(prm.solver_type == TSPH_VFSPH ? f->sph_compression : f->sph_density);
   hipLaunchKernelGGL(k_surface_tension<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n,
                      (const V4<T>*)f->pos, (const T*)f->mass, psi, (const T*)f->rest_density,
@@ -744,6 +833,7 @@ int fbm_round(const TsphParams& prm, TsphObject* f, TsphControl* ctrl, void* ws,
   hipLaunchKernelGGL(k_fbm_check<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n,
                      (const T*)f->volume_frac, (const T*)f->volume_frac_tmp, f->flag, L.partial,
                      ctrl);
+  hipLaunchKernelGGL(k_fbm_check_finish, dim3(1), dim3(kBlock), 0, st, L.partial, nb, ctrl);
   TSPH_LAUNCH_CHECK("fbm_round");
   return 0;
 }
@@ -760,6 +850,8 @@ int tail(const TsphParams& prm, TsphObject* f, TsphControl* ctrl, int fbm, void*
                      (V4<T>*)f->vel, (const V4<T>*)f->vel_adv, (T*)f->mass,
                      (T*)f->rest_density, (const T*)f->rest_volume, (T*)f->volume_frac,
                      (const T*)f->volume_frac_tmp, f->flag, L.partial, ctrl);
+  hipLaunchKernelGGL(k_finalize<T>, dim3(1), dim3(kBlock), 0, st, k, (int)kFinEnergy, L.partial, nb,
+                     f->n, ctrl);
   TSPH_LAUNCH_CHECK("tail");
   return 0;
 }
@@ -768,8 +860,8 @@ template int density_alpha<float>(const TsphParams&, TsphObject*, TsphObject*, T
 template int density_alpha<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template int solver_begin<float>(const TsphParams&, TsphControl*, int, hipStream_t);
 template int solver_begin<double>(const TsphParams&, TsphControl*, int, hipStream_t);
-template int solver_iteration<float>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, void*, size_t, hipStream_t);
-template int solver_iteration<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, void*, size_t, hipStream_t);
+template int solver_half<float>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, int, void*, size_t, hipStream_t);
+template int solver_half<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, int, void*, size_t, hipStream_t);
 template int advect<float>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template int advect<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template int fbm_round<float>(const TsphParams&, TsphObject*, TsphControl*, void*, size_t, hipStream_t);

@@ -71,10 +71,23 @@ class Simulation:
             raise Exception(f"solver_iter_threshold_max must be < {_lib.TSPH_MAX_ITER_LOG}")
         self.set_dt(config.dt)
         self.time_count = 0.0
+        # optional per-call HIP-event timing: a list receiving (name, start, end)
+        self.timer = None
 
     # ----------------------------------------------------------------- helpers
     def stream_ptr(self):
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _call(self, name, fn, *args):
+        if self.timer is None:
+            _lib.check(fn(*args), name)
+            return
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        _lib.check(fn(*args), name)
+        e1.record()
+        self.timer.append((name, e0, e1))
 
     def ctrl_ptr(self):
         return ctypes.c_void_p(self.ctrl.data_ptr())
@@ -159,8 +172,8 @@ class Simulation:
         chunk = max(self.pred[solve], 1)
         while k < kmax:
             for _ in range(min(chunk, kmax - k)):
-                rc = self.lib.tsph_solver_iteration(P, f, b, self.ctrl_ptr(), k, ws, wsb, st)
-                _lib.check(rc, "tsph_solver_iteration")
+                self._call("solver_psi", self.lib.tsph_solver_psi, P, f, b, self.ctrl_ptr(), k, ws, wsb, st)
+                self._call("solver_update", self.lib.tsph_solver_update, P, f, b, self.ctrl_ptr(), k, ws, wsb, st)
                 k += 1
             it, stop, _ = self._stop_state()
             if stop:
@@ -178,24 +191,23 @@ class Simulation:
         st = self.stream_ptr()
         if self.bound_dirty:
             self._sort_bound()
-        rc = self.lib.tsph_sort_object(P, ctypes.byref(self.fluid.struct()), self.ctrl_ptr(),
-                                       1 if self.cfl else 0, ws, wsb, st)
-        _lib.check(rc, "tsph_sort_object(fluid)")
+        self._call("sort", self.lib.tsph_sort_object, P, ctypes.byref(self.fluid.struct()), self.ctrl_ptr(),
+                   1 if self.cfl else 0, ws, wsb, st)
         self.fluid.adopt_swaps()
         f, b = ctypes.byref(self.fluid.struct()), ctypes.byref(self.bound.struct())
-        _lib.check(self.lib.tsph_density_alpha(P, f, b, self.ctrl_ptr(), ws, wsb, st), "tsph_density_alpha")
+        self._call("density_alpha", self.lib.tsph_density_alpha, P, f, b, self.ctrl_ptr(), ws, wsb, st)
         div_it = self._solve(0)
-        _lib.check(self.lib.tsph_advect(P, f, b, self.ctrl_ptr(), ws, wsb, st), "tsph_advect")
+        self._call("advect", self.lib.tsph_advect, P, f, b, self.ctrl_ptr(), ws, wsb, st)
         inc_it = self._solve(1)
         if self.fbm:
             rounds = 0
             while rounds < 1000:
-                _lib.check(self.lib.tsph_fbm_round(P, f, self.ctrl_ptr(), ws, wsb, st), "tsph_fbm_round")
+                self._call("fbm_round", self.lib.tsph_fbm_round, P, f, self.ctrl_ptr(), ws, wsb, st)
                 rounds += 1
                 _, _, gflag = self._stop_state()
                 if not gflag:
                     break
-        _lib.check(self.lib.tsph_tail(P, f, self.ctrl_ptr(), 1 if self.fbm else 0, ws, wsb, st), "tsph_tail")
+        self._call("tail", self.lib.tsph_tail, P, f, self.ctrl_ptr(), 1 if self.fbm else 0, ws, wsb, st)
         if stats:
             return StepStats(self.control(), self.precision)
         return div_it, inc_it
