@@ -6,13 +6,19 @@ Jacobi solves to their stop rule, advection, tail) over the 1,000,000-fluid /
 inputs resident in HBM.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+  python bench.py --copy-calibration MIB   # plain copy kernel of known size
 
 Roofline: algorithmic bytes of the dominant kernel per launch (per-particle
-figures of the minimal-pass schedule) divided by its average launch time from
-HIP events recorded on the launch stream inside the timed region.
+figures of the minimal-pass schedule) divided by its average launch duration
+from HIP events recorded on the launch stream inside the timed region.
+`traffic` is the dominant kernel's HBM bytes per launch from FETCH_SIZE and
+WRITE_SIZE counter runs, each scaled by what the same counter reads for the
+copy kernel of --copy-calibration; the figures are read from
+profiles/traffic_dambreak_3d_1m.json (null when that file is absent).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -42,6 +48,94 @@ ALG_BYTES = {
 def step_bytes(nf, nb, nc, I):
     """B_step of the minimal-pass schedule (3D, FP32)."""
     return nf * ((10 * V + 11 * S) + I * (5 * V + 7 * S)) + nb * (2 * V + 4 * S) * (1 + I) + 16 * nc
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of a bench kernel group from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE, KiB per dispatch, copy-calibrated)."""
+    path = os.path.join(ROOT, "profiles", "traffic_" + WORKLOAD + ".json")
+    if not os.path.exists(path):
+        return None
+    d = json.load(open(path))
+    g = d.get("groups", {}).get(kernel)
+    if g is None:
+        return None
+    return int(g["fetch_kib"] * 1024 * d["fetch_factor"] + g["write_kib"] * 1024 * d["write_factor"])
+
+
+def copy_calibration(mib, reps=5):
+    """A plain elementwise copy (read N bytes, write N bytes) for scaling the
+    FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, device="cuda", dtype=torch.float32)
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 4, "reps": reps,
+                      "kernel": "vectorized_elementwise_kernel (MulFunctor)"}), flush=True)
+
+
+def adjoint_leg(scen, steps, every):
+    """fwd vs fwd+adjoint on the same workload (fixed dt, which the CFL
+    condition never lowers here): `steps` plain forward steps, then a
+    differentiable rollout of `steps` steps (checkpoint every `every`) with
+    the backward pass of a position loss; wall time of each, ms per step."""
+    import torch
+    from tichi_sph_amd import load_simulation
+    from tichi_sph_amd.adjoint import DiffRunner, rollout
+
+    sim = load_simulation(os.path.join(scen, WORKLOAD + ".config.json"), os.path.join(scen, WORKLOAD + ".json"),
+                          cfl=False)
+    r = DiffRunner(sim)
+    st0 = r.checkpoint()
+    nf, d = sim.fluid.part_num, sim.config.dim
+    dev, dt_ = sim.device, sim.fluid.dtype
+    x_init = r.positions_ref().clone()
+    mu0, gam0 = sim.params.dynamic_viscosity, sim.params.surface_tension_gamma
+
+    def diff_run(k):
+        r.restore(st0)
+        v = torch.zeros(nf, d, dtype=dt_, device=dev, requires_grad=True)
+        mu = torch.tensor(mu0, dtype=dt_, device=dev, requires_grad=True)
+        gam = torch.tensor(gam0, dtype=dt_, device=dev, requires_grad=True)
+        x, _ = rollout(r, v, mu, gam, k, every)
+        ((x - x_init) ** 2).sum(1).mean().backward()
+        return v, mu, gam
+
+    diff_run(2)  # warm-up (allocator, first launches)
+    r.restore(st0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.step()
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    sim.timer = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    v, mu, gam = diff_run(steps)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    per = {}
+    for name, e0, e1 in sim.timer:
+        dd = per.setdefault(name, [0.0, 0])
+        dd[0] += e0.elapsed_time(e1)
+        dd[1] += 1
+    sim.timer = None
+    r.restore(st0)
+    tp = r.step(tape=True)
+    tape_mb = tp.nbytes() / 1e6
+    del tp
+    ms_f, ms_a = t_fwd / steps * 1e3, t_adj / steps * 1e3
+    return {"steps": steps, "checkpoint_every": every, "ms_per_step_fwd": round(ms_f, 4),
+            "ms_per_step_fwd_adj": round(ms_a, 4), "ratio": round(ms_a / ms_f, 3),
+            "value_fwd_adj": round(nf * steps / t_adj, 1),
+            "grad_finite": bool(torch.isfinite(v.grad).all().item() and math.isfinite(float(mu.grad))
+                                and math.isfinite(float(gam.grad))),
+            "tape_MB_per_step": round(tape_mb, 1),
+            "kernel_ms_per_step": {k: round(x[0] / steps, 4) for k, x in sorted(per.items())}}
 
 
 def cpu_baseline(steps=2):
@@ -81,11 +175,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timers", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
+    ap.add_argument("--adjoint-steps", type=int, default=20)
+    ap.add_argument("--checkpoint-every", type=int, default=10)
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus must be 1)",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     import torch
     from tichi_sph_amd import load_simulation
@@ -121,7 +222,7 @@ def main():
         alg = ALG_BYTES[dom](nf, nb)
         ach = alg / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom), "kernel": dom,
                 "avg_launch_us": round(avg_ms * 1e3, 2), "alg_bytes_per_launch": alg}
     B = step_bytes(nf, nb, nc, I_mean)
     step_gbs = B / (ms_per_step * 1e-3) / 1e9
@@ -140,6 +241,10 @@ def main():
                        "max_div": max(a for a, _ in iters), "max_incom": max(b for _, b in iters)},
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in sorted(per.items())},
     }
+    if not args.no_adjoint:
+        del sim
+        torch.cuda.empty_cache()
+        out["adjoint"] = adjoint_leg(scen, args.adjoint_steps, args.checkpoint_every)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

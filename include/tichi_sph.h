@@ -229,6 +229,78 @@ int tsph_tail(const TsphParams* p, TsphObject* fluid, TsphControl* ctrl, int32_t
               void* workspace, size_t ws_bytes, void* stream);
 
 /* ----------------------------------------------------------------------
+ * Reverse mode (adjoint) of the DFSPH step.  The reference has no
+ * autodiff; these entries differentiate the fused step above (sph_step,
+ * main.py:175-256) for one phase, DFSPH, fixed dt, no FBM, with the Jacobi
+ * iteration counts and the clamp / neighbour sets of the recorded forward
+ * step held fixed.  Every reverse sweep is the transpose of a forward sweep
+ * written as a gather over the same symmetric neighbour lists, so no
+ * atomics are used and results are deterministic.
+ * ---------------------------------------------------------------------- */
+
+/* Adjoint buffers of one step, all [capacity] in the step's cell order.
+ * V4 arrays hold 4 values of the precision's float type, T arrays one. */
+typedef struct TsphAdjoint {
+  void *xbar;         /* V4: d loss / d pos at the step's start (accumulated) */
+  void *vbar;         /* V4: d loss / d (velocity state being reversed), read */
+  void *vbar_out;     /* V4: written by the sweeps that move the velocity state back */
+  void *rhobar;       /* T: d loss / d sph_density (accumulated) */
+  void *alphabar;     /* T: d loss / d alpha (accumulated) */
+  void *psibar;       /* T: d loss / d psi_adv before the clamp, per iteration */
+  void *nbar;         /* V4: d loss / d normal */
+  void *a1bar;        /* V4: d loss / d alpha_1 (xyz) and d alpha_2 (w) */
+  void *tape_vel_adv; /* V4: recorded vel_adv entering the iteration being reversed */
+  void *tape_kappa;   /* T: recorded kappa (psi_adv / alpha) of that iteration */
+  void *tape_vel;     /* V4: recorded velocity after the divergence solve */
+  void *tape_normal;  /* V4: recorded surface-tension normals */
+  double *grad;       /* [2]: accumulated d loss / d dynamic_viscosity, d surface_tension_gamma */
+} TsphAdjoint;
+
+/* tsph_solver_psi that also records vel_adv entering iteration k and the
+ * iteration's kappa into the two [capacity] tape arrays (the forward half of
+ * tsph_adj_solver_iteration). */
+int tsph_solver_psi_taped(const TsphParams* p, TsphObject* fluid, TsphObject* bound,
+                          TsphControl* ctrl, int32_t k, void* tape_vel_adv, void* tape_kappa,
+                          void* workspace, size_t ws_bytes, void* stream);
+
+/* The fluid half of tsph_density_alpha alone: neighbour lists, density and
+ * alpha of the fluid recomputed (bit-identically) from recorded sorted
+ * positions, masses and cell table, so that per-step lists are recomputed
+ * instead of stored. */
+int tsph_relist(const TsphParams* p, TsphObject* fluid, TsphObject* bound, TsphControl* ctrl,
+                void* workspace, size_t ws_bytes, void* stream);
+
+/* Opens the reverse of one step (tail, SPH_update_pos sph.py:265-269):
+ * reads d loss / d pos and d loss / d vel after the step from V4 arrays in
+ * reference (pid) order, gathers them into the step's cell order with
+ * pid[], and clears the accumulated density / alpha adjoints. */
+int tsph_adj_begin(const TsphParams* p, TsphObject* fluid, TsphControl* ctrl, TsphAdjoint* adj,
+                   const int32_t* pid, const void* xbar_ref, const void* vbar_ref, void* stream);
+
+/* Reverse of one Jacobi iteration (IPPE_adv_psi + IPPE_psi_adv_non_negative
+ * + IPPE_update_vel_adv, sph.py:196-244) whose record is adj->tape_vel_adv /
+ * adj->tape_kappa: vbar -> vbar_out, accumulating xbar, rhobar, alphabar. */
+int tsph_adj_solver_iteration(const TsphParams* p, TsphObject* fluid, TsphObject* bound,
+                              TsphControl* ctrl, TsphAdjoint* adj, void* workspace,
+                              size_t ws_bytes, void* stream);
+
+/* Reverse of the advection (viscosity, normals, surface tension and
+ * vel_adv = vel + dt * acce_adv; sph.py:108-166, 247-250): vbar -> vbar_out,
+ * accumulating xbar, rhobar and adj->grad. */
+int tsph_adj_advect(const TsphParams* p, TsphObject* fluid, TsphControl* ctrl, TsphAdjoint* adj,
+                    void* workspace, size_t ws_bytes, void* stream);
+
+/* Reverse of density and alpha (SPH_prepare_attr, _alpha_1, _alpha_2,
+ * _alpha; sph.py:50-105): accumulates xbar from rhobar and alphabar. */
+int tsph_adj_density_alpha(const TsphParams* p, TsphObject* fluid, TsphObject* bound,
+                           TsphAdjoint* adj, void* stream);
+
+/* Closes the reverse of one step: scatters xbar and vbar (d loss / d pos and
+ * d loss / d vel at the step's start) to reference order with pid[]. */
+int tsph_adj_end(const TsphParams* p, TsphObject* fluid, TsphAdjoint* adj, const int32_t* pid,
+                 void* xbar_ref, void* vbar_ref, void* stream);
+
+/* ----------------------------------------------------------------------
  * Reference kernel surface: the sph.py kernels one by one, with the
  * reference's (ngrid, obj, nobj) signature, walking the Ngrid built by
  * tsph_ngrid_build exactly as the reference does.  `kernel` is one of the

@@ -70,6 +70,14 @@ class TsphControl(ctypes.Structure):
 
 
 # Reference kernel ids (sph.py), include/tichi_sph.h
+_ADJ_PTRS = ["xbar", "vbar", "vbar_out", "rhobar", "alphabar", "psibar", "nbar", "a1bar",
+             "tape_vel_adv", "tape_kappa", "tape_vel", "tape_normal", "grad"]
+
+
+class TsphAdjoint(ctypes.Structure):
+    _fields_ = [(name, _vp) for name in _ADJ_PTRS]
+
+
 KERNEL_IDS = {
     "SPH_clean_value": 0, "cfl_condition": 1, "SPH_prepare_attr": 2,
     "SPH_prepare_alpha_1": 3, "SPH_prepare_alpha_2": 4, "SPH_prepare_alpha": 5,
@@ -85,8 +93,10 @@ KERNEL_IDS = {
 EXPORTED = [
     "tsph_last_error", "tsph_abi_version", "tsph_control_bytes", "tsph_workspace_bytes",
     "tsph_sort_object", "tsph_ngrid_build", "tsph_neighbor_lists", "tsph_density_alpha",
-    "tsph_solver_begin", "tsph_solver_iteration", "tsph_solver_psi", "tsph_solver_update", "tsph_advect", "tsph_fbm_round", "tsph_tail",
-    "tsph_ref_kernel",
+    "tsph_solver_begin", "tsph_solver_iteration", "tsph_solver_psi", "tsph_solver_update", "tsph_advect",
+    "tsph_fbm_round", "tsph_tail",
+    "tsph_ref_kernel", "tsph_solver_psi_taped", "tsph_relist", "tsph_adj_begin",
+    "tsph_adj_solver_iteration", "tsph_adj_advect", "tsph_adj_density_alpha", "tsph_adj_end",
 ]
 
 _lib = None
@@ -116,6 +126,7 @@ def load(path=None):
     lib.tsph_workspace_bytes.restype = ctypes.c_size_t
     lib.tsph_workspace_bytes.argtypes = [ctypes.POINTER(TsphParams), _i32]
     P, O, C = ctypes.POINTER(TsphParams), ctypes.POINTER(TsphObject), _vp
+    A = ctypes.POINTER(TsphAdjoint)
     sz = ctypes.c_size_t
     sigs = {
         "tsph_sort_object": [P, O, C, _i32, _vp, sz, _vp],
@@ -130,6 +141,13 @@ def load(path=None):
         "tsph_fbm_round": [P, O, C, _vp, sz, _vp],
         "tsph_tail": [P, O, C, _i32, _vp, sz, _vp],
         "tsph_ref_kernel": [P, _i32, O, O, _vp, _vp, _vp, _vp, C, _vp, sz, _vp],
+        "tsph_solver_psi_taped": [P, O, O, C, _i32, _vp, _vp, _vp, sz, _vp],
+        "tsph_relist": [P, O, O, C, _vp, sz, _vp],
+        "tsph_adj_begin": [P, O, C, A, _vp, _vp, _vp, _vp],
+        "tsph_adj_solver_iteration": [P, O, O, C, A, _vp, sz, _vp],
+        "tsph_adj_advect": [P, O, C, A, _vp, sz, _vp],
+        "tsph_adj_density_alpha": [P, O, O, A, _vp],
+        "tsph_adj_end": [P, O, A, _vp, _vp, _vp, _vp],
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)

@@ -1,0 +1,268 @@
+"""Differentiable DFSPH rollouts: hand-written HIP adjoint kernels behind
+`torch.autograd.Function`.
+
+The reference has no autodiff (no `needs_grad`, `ti.Tape` or `ti.ad` in
+sph.py / main.py).  This module differentiates the fused step of
+`Simulation.sph_step()` (main.py:175-256) for one-phase DFSPH with a fixed
+dt and no FBM.  The derivative is that of the recorded forward step: the
+Jacobi iteration counts, the psi clamps (sph.py:219-226), the alpha clamp
+(sph.py:103-105) and the neighbour sets are held at their recorded values;
+between such events the step is smooth (the cubic spline is C^1 with
+W'(h) = W''(h) = 0) and the derivative is exact.
+
+Memory.  The forward pass keeps only checkpoints: the persistent state
+(pos, vel, mass, rest density, rest volume, volume fractions, pid and the
+control block) every `every` steps.  The backward pass replays each segment
+from its checkpoint while recording a tape per step, then reverses the
+segment step by step.  A tape holds the sorted positions, masses, rest
+densities, pids and cell table, vel_adv and kappa of every Jacobi iteration,
+and the advection's velocity and normals.  Neighbour lists, densities and
+alpha factors are not stored: `tsph_relist` recomputes them bit-identically
+from the recorded positions.
+
+    runner = DiffRunner(sim)
+    x, v = rollout(runner, v0, mu, gamma, steps=200, every=20)
+    loss = ((x - target) ** 2).sum(1).mean()
+    loss.backward()          # v0.grad, mu.grad, gamma.grad
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+
+
+class StepTape:
+    """Everything the reverse of one step reads that is not recomputed."""
+
+    __slots__ = ("pos", "mass", "rest_density", "pid", "cell_start", "vel", "normal", "iters")
+
+    def __init__(self):
+        self.iters = ([], [])  # per solve: [(vel_adv entering it, kappa), ...]
+
+    def nbytes(self):
+        ts = [self.pos, self.mass, self.rest_density, self.pid, self.cell_start, self.vel, self.normal]
+        ts += [t for s in self.iters for pair in s for t in pair]
+        return sum(t.numel() * t.element_size() for t in ts)
+
+
+class DiffRunner:
+    """Drives a `Simulation` forward with optional tapes and backward through
+    the adjoint entry points of the C ABI."""
+
+    def __init__(self, sim):
+        cfg = sim.config
+        if cfg.solver_type != "DFSPH" or cfg.phase_num != 1:
+            raise ValueError("the adjoint supports one-phase DFSPH scenes only")
+        if sim.fbm:
+            raise ValueError("the adjoint does not differentiate FBM rounds")
+        if sim.cfl:
+            raise ValueError("the adjoint needs a fixed dt: build the Simulation with cfl=False")
+        self.sim = sim
+        self.lib = sim.lib
+        f = sim.fluid
+        cap = f.t["pos"].shape[0]
+        dt_, dev = f.dtype, sim.device
+        self.buf = {name: torch.zeros(cap, 4, dtype=dt_, device=dev)
+                    for name in ("xbar", "vbar", "vbar_out", "nbar", "a1bar")}
+        for name in ("rhobar", "alphabar", "psibar"):
+            self.buf[name] = torch.zeros(cap, dtype=dt_, device=dev)
+        self.grad = torch.zeros(2, dtype=torch.float64, device=dev)
+        self.adj = _lib.TsphAdjoint()
+        for name, t in self.buf.items():
+            setattr(self.adj, name, t.data_ptr())
+        self.adj.grad = self.grad.data_ptr()
+        self.force = None     # optional list of (div_iter, incom_iter) per step to replay
+        self.step_index = 0
+        self.iter_log = []    # (div_iter, incom_iter) of every forward step since the last reset
+
+    # -------------------------------------------------------------- params
+    def set_params(self, mu=None, gamma=None):
+        if mu is not None:
+            self.sim.params.dynamic_viscosity = float(mu)
+        if gamma is not None:
+            self.sim.params.surface_tension_gamma = float(gamma)
+
+    # --------------------------------------------------------------- state
+    def checkpoint(self):
+        f = self.sim.fluid
+        n = f.part_num
+        st = {name: f.t[name][:n].clone() for name in
+              ("pos", "vel", "mass", "rest_density", "rest_volume", "volume_frac", "pid")}
+        st["ctrl"] = self.sim.ctrl.clone()
+        st["step_index"] = self.step_index
+        return st
+
+    def restore(self, st):
+        f = self.sim.fluid
+        n = f.part_num
+        for name in ("pos", "vel", "mass", "rest_density", "rest_volume", "volume_frac", "pid"):
+            f.t[name][:n].copy_(st[name])
+        self.sim.ctrl.copy_(st["ctrl"])
+        self.step_index = st["step_index"]
+
+    def set_velocity_ref(self, v):
+        """Fluid velocities from an [n, dim] tensor in reference (pid) order."""
+        f = self.sim.fluid
+        n, d = f.part_num, self.sim.config.dim
+        pid = f.t["pid"][:n].long()
+        f.t["vel"][:n, :d].copy_(v.to(f.dtype)[pid])
+
+    def positions_ref(self):
+        f = self.sim.fluid
+        n, d = f.part_num, self.sim.config.dim
+        out = torch.empty(n, d, dtype=f.dtype, device=self.sim.device)
+        out[f.t["pid"][:n].long()] = f.t["pos"][:n, :d]
+        return out
+
+    def velocities_ref(self):
+        f = self.sim.fluid
+        n, d = f.part_num, self.sim.config.dim
+        out = torch.empty(n, d, dtype=f.dtype, device=self.sim.device)
+        out[f.t["pid"][:n].long()] = f.t["vel"][:n, :d]
+        return out
+
+    # ------------------------------------------------------------- forward
+    def step(self, tape=False):
+        """One fused step (Simulation.sph_step without CFL / FBM); with
+        tape=True returns the StepTape of the step."""
+        sim = self.sim
+        f, b = sim.fluid, sim.bound
+        P = ctypes.byref(sim.params)
+        ws, wsb = sim._ws()
+        stp = sim.stream_ptr()
+        force = self.force[self.step_index] if self.force is not None else (None, None)
+        if sim.bound_dirty:
+            sim._sort_bound()
+        sim._call("sort", sim.lib.tsph_sort_object, P, ctypes.byref(f.struct()), sim.ctrl_ptr(), 0, ws, wsb, stp)
+        f.adopt_swaps()
+        tp = None
+        n = f.part_num
+        if tape:
+            tp = StepTape()
+            tp.pos = f.t["pos"][:n].clone()
+            tp.mass = f.t["mass"][:n].clone()
+            tp.rest_density = f.t["rest_density"][:n].clone()
+            tp.pid = f.t["pid"][:n].clone()
+            tp.cell_start = f.t["cell_start"].clone()
+        fs, bs = ctypes.byref(f.struct()), ctypes.byref(b.struct())
+        sim._call("density_alpha", sim.lib.tsph_density_alpha, P, fs, bs, sim.ctrl_ptr(), ws, wsb, stp)
+        d_it = sim._solve(0, tape=tp.iters[0] if tape else None, force=force[0])
+        sim._call("advect", sim.lib.tsph_advect, P, fs, bs, sim.ctrl_ptr(), ws, wsb, stp)
+        if tape:
+            tp.vel = f.t["vel"][:n].clone()
+            tp.normal = f.t["normal"][:n].clone()
+        i_it = sim._solve(1, tape=tp.iters[1] if tape else None, force=force[1])
+        sim._call("tail", sim.lib.tsph_tail, P, fs, sim.ctrl_ptr(), 0, ws, wsb, stp)
+        self.iter_log.append((d_it, i_it))
+        self.step_index += 1
+        return tp
+
+    # ------------------------------------------------------------ backward
+    def reverse(self, tp, xbar_ref, vbar_ref):
+        """Reverse of one recorded step.  xbar_ref / vbar_ref ([n, 4], pid
+        order) hold d loss / d (pos, vel) after the step on entry and before
+        it on return.  adj.grad accumulates d loss / d (mu, gamma)."""
+        sim = self.sim
+        f, b = sim.fluid, sim.bound
+        n = f.part_num
+        P = ctypes.byref(sim.params)
+        ws, wsb = sim._ws()
+        stp = sim.stream_ptr()
+        f.t["pos"][:n].copy_(tp.pos)
+        f.t["mass"][:n].copy_(tp.mass)
+        f.t["rest_density"][:n].copy_(tp.rest_density)
+        f.t["cell_start"].copy_(tp.cell_start)
+        fs, bs = ctypes.byref(f.struct()), ctypes.byref(b.struct())
+        A = ctypes.byref(self.adj)
+        vp = ctypes.c_void_p
+        sim._call("relist", self.lib.tsph_relist, P, fs, bs, sim.ctrl_ptr(), ws, wsb, stp)
+        sim._call("adj_begin", self.lib.tsph_adj_begin, P, fs, sim.ctrl_ptr(), A, vp(tp.pid.data_ptr()),
+                  vp(xbar_ref.data_ptr()), vp(vbar_ref.data_ptr()), stp)
+
+        def iterations(pairs):
+            for tv, tk in reversed(pairs):
+                self.adj.tape_vel_adv = tv.data_ptr()
+                self.adj.tape_kappa = tk.data_ptr()
+                sim._call("adj_solver_iteration", self.lib.tsph_adj_solver_iteration, P, fs, bs,
+                          sim.ctrl_ptr(), A, ws, wsb, stp)
+                self._swap_v()
+
+        iterations(tp.iters[1])
+        self.adj.tape_vel = tp.vel.data_ptr()
+        self.adj.tape_normal = tp.normal.data_ptr()
+        sim._call("adj_advect", self.lib.tsph_adj_advect, P, fs, sim.ctrl_ptr(), A, ws, wsb, stp)
+        self._swap_v()
+        iterations(tp.iters[0])
+        sim._call("adj_density_alpha", self.lib.tsph_adj_density_alpha, P, fs, bs, A, stp)
+        sim._call("adj_end", self.lib.tsph_adj_end, P, fs, A, vp(tp.pid.data_ptr()),
+                  vp(xbar_ref.data_ptr()), vp(vbar_ref.data_ptr()), stp)
+
+    def _swap_v(self):
+        self.buf["vbar"], self.buf["vbar_out"] = self.buf["vbar_out"], self.buf["vbar"]
+        self.adj.vbar = self.buf["vbar"].data_ptr()
+        self.adj.vbar_out = self.buf["vbar_out"].data_ptr()
+
+
+class _Rollout(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, v0, mu, gamma, runner, steps, every):
+        runner.set_params(mu.item(), gamma.item())
+        runner.set_velocity_ref(v0.detach())
+        runner.step_index = 0
+        runner.iter_log = []
+        ckpts = []
+        for t in range(steps):
+            if t % every == 0:
+                ckpts.append(runner.checkpoint())
+            runner.step()
+        ctx.runner, ctx.ckpts, ctx.steps, ctx.every = runner, ckpts, steps, every
+        ctx.mu, ctx.gamma = float(mu.item()), float(gamma.item())
+        ctx.v0_dtype = v0.dtype
+        return runner.positions_ref(), runner.velocities_ref()
+
+    @staticmethod
+    def backward(ctx, gx, gv):
+        runner = ctx.runner
+        sim = runner.sim
+        f = sim.fluid
+        n, d = f.part_num, sim.config.dim
+        final = runner.checkpoint()
+        runner.set_params(ctx.mu, ctx.gamma)
+        xbar = torch.zeros(n, 4, dtype=f.dtype, device=sim.device)
+        vbar = torch.zeros(n, 4, dtype=f.dtype, device=sim.device)
+        if gx is not None:
+            xbar[:, :d] = gx.to(f.dtype)
+        if gv is not None:
+            vbar[:, :d] = gv.to(f.dtype)
+        runner.grad.zero_()
+        saved_force, saved_log = runner.force, runner.iter_log
+        runner.force = list(saved_log)  # replay the forward's iteration counts exactly
+        runner.iter_log = []
+        for s in reversed(range(len(ctx.ckpts))):
+            t0 = s * ctx.every
+            t1 = min(t0 + ctx.every, ctx.steps)
+            runner.restore(ctx.ckpts[s])
+            tapes = [runner.step(tape=True) for _ in range(t0, t1)]
+            for tp in reversed(tapes):
+                runner.reverse(tp, xbar, vbar)
+            del tapes
+        runner.force, runner.iter_log = saved_force, saved_log
+        runner.restore(final)
+        g = runner.grad.to(torch.float64)
+        gmu = g[0].to(ctx.v0_dtype)
+        ggam = g[1].to(ctx.v0_dtype)
+        return vbar[:, :d].to(ctx.v0_dtype), gmu, ggam, None, None, None
+
+
+def rollout(runner, v0, mu, gamma, steps, every=20):
+    """Differentiable rollout of `steps` fused DFSPH steps from the runner's
+    current state with initial fluid velocity v0 ([n, dim], reference
+    order), dynamic viscosity mu and surface-tension coefficient gamma (0-d
+    tensors).  Returns final (positions, velocities) in reference order.
+    The state is checkpointed every `every` steps."""
+    if not torch.is_tensor(mu):
+        mu = torch.tensor(float(mu), dtype=v0.dtype, device=v0.device)
+    if not torch.is_tensor(gamma):
+        gamma = torch.tensor(float(gamma), dtype=v0.dtype, device=v0.device)
+    return _Rollout.apply(v0, mu, gamma, runner, int(steps), max(int(every), 1))

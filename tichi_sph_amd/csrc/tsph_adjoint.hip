@@ -1,0 +1,610 @@
+// Reverse mode of the fused DFSPH step (one phase, fixed dt, no FBM).
+//
+// The forward step (tsph_step.hip) is a chain of neighbour sums
+//   out_i = sum_j F(x_ij, a_i, a_j)
+// over the symmetric in-range lists (j in N(i) <=> i in N(j) for fluid
+// pairs).  The adjoint of such a sum with respect to particle l collects
+// the pair (l, j) from l's own equation and the pair (j, l) from j's
+// equation; both are visible from l's list, so every reverse sweep is a
+// gather over l's neighbours: no scatter, no atomics, a fixed order.
+//
+// Notation.  x = x_l - x_j, r = |x|, g = W'(r)/r so that grad W = g x, and
+// J = dG/dx for G(x) = g x:  J w = g w + hq (x.w) x with hq = (W'' - g)/r^2.
+// For a term c . G(x_lj) in l's equation and c' . G(x_jl) in j's equation,
+// d/dx_l = J (c - c').
+//
+// The forward schedule being reversed (sph.py line numbers):
+//   pass A: rho, alpha_1, alpha_2, alpha (50-105), vel_adv = vel (253-256)
+//   divergence solve iterations (196-244)
+//   advection: viscosity, normals, surface tension, vel_adv = vel + dt a
+//     (108-166, 247-250)
+//   density solve iterations (196-244)
+//   tail: vel = vel_adv, pos += dt vel (259-269)
+#include "tsph_sweep.h"
+
+namespace tsph {
+
+// g, hq and W'' of the cubic spline at r^2 (sph_util.py:28-37 differentiated
+// once more): q <= 1/2: W' = 6 sig/h (3q^2 - 2q), W'' = 6 sig/h^2 (6q - 2);
+// 1/2 < q < 1: W' = -6 sig/h (1-q)^2, W'' = 12 sig/h^2 (1-q).
+template <class T>
+struct D2 {
+  T g, hq, w2, r;
+};
+template <class T>
+__device__ __forceinline__ D2<T> kern_d2(const KP<T>& k, T r2) {
+  T rinv = f_rsqrt(r2);
+  T r = r2 * rinv;
+  T q = r * k.inv_h;
+  T u = T(1) - q;
+  D2<T> d;
+  d.r = r;
+  if (q <= T(0.5)) {
+    d.g = k.wg_a * (T(3) * q - T(2));
+    d.w2 = k.wg_a * (T(6) * q - T(2));
+  } else if (q < T(1)) {
+    d.g = -k.wg_b * (u * u) * rinv;
+    d.w2 = T(2) * k.wg_a * u;
+  } else {
+    d.g = T(0);
+    d.w2 = T(0);
+  }
+  d.hq = (d.w2 - d.g) * (rinv * rinv);
+  return d;
+}
+
+// Akinci cohesion C(r) and C'(r) (sph_util.py:4-13).
+template <class T>
+__device__ __forceinline__ void coh_d1(const KP<T>& k, T r, T* C, T* Cp) {
+  T q = r * k.inv_h;
+  T u = T(1) - q;
+  T c = T(0), dc = T(0);
+  if (q <= T(0.5)) {
+    c = T(2) * (u * u * u) * (q * q * q) - T(1.0 / 64);
+    dc = T(6) * (u * u) * (q * q) * (T(1) - T(2) * q);
+  } else if (q < T(1)) {
+    c = (u * u * u) * (q * q * q);
+    dc = T(3) * (u * u) * (q * q) * (T(1) - T(2) * q);
+  }
+  *C = c * k.c_coh;
+  *Cp = dc * k.c_coh * k.inv_h;
+}
+
+template <class T>
+__device__ __forceinline__ V4<T> v4(T x, T y, T z, T w = T(0)) {
+  return V4<T>{x, y, z, w};
+}
+template <class T>
+__device__ __forceinline__ T dot3(const V4<T>& a, const V4<T>& b) {
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+// acc += J(x) c
+template <class T>
+__device__ __forceinline__ void add_J(T& ax, T& ay, T& az, const D2<T>& d, T dx, T dy, T dz,
+                                      T cx, T cy, T cz) {
+  T s = d.hq * (dx * cx + dy * cy + dz * cz);
+  ax += d.g * cx + s * dx;
+  ay += d.g * cy + s * dy;
+  az += d.g * cz + s * dz;
+}
+
+// ---------------------------------------------------------------------------
+// Tail reverse: pos' = pos + dt vel_adv, vel' = vel_adv (sph.py:259-269).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_adj_begin(int n, const int* __restrict__ pid, const V4<T>* __restrict__ xref,
+                const V4<T>* __restrict__ vref, V4<T>* __restrict__ xbar, V4<T>* __restrict__ vbar,
+                T* __restrict__ rhobar, T* __restrict__ alphabar, const TsphControl* ctrl) {
+  int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const T dt = (T)ctrl->dt;
+  int p = pid[i];
+  V4<T> xb = xref[p], vb = vref[p];
+  xbar[i] = v4(xb.x, xb.y, xb.z);
+  vbar[i] = v4(vb.x + dt * xb.x, vb.y + dt * xb.y, vb.z + dt * xb.z);
+  rhobar[i] = T(0);
+  alphabar[i] = T(0);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_adj_end(int n, const int* __restrict__ pid, const V4<T>* __restrict__ xbar,
+              const V4<T>* __restrict__ vbar, V4<T>* __restrict__ xref, V4<T>* __restrict__ vref) {
+  int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  int p = pid[i];
+  V4<T> xb = xbar[i], vb = vbar[i];
+  xref[p] = v4(xb.x, xb.y, xb.z);
+  vref[p] = v4(vb.x, vb.y, vb.z);
+}
+
+// ---------------------------------------------------------------------------
+// Jacobi iteration reverse, half 1: IPPE_update_vel_adv (sph.py:229-244)
+//   v'_l = v_l - (1/dt)/m_l sum_j (kappa_l X_j + kappa_j X_l) G(x_lj)
+// gives, with u = d loss / d v' and w = u / m,
+//   kappa_bar_l = -(1/dt) sum_j X_j G(x_lj) . (w_l - w_j)
+//   x_bar_l    += -(1/dt) sum_j J (kappa_l X_j + kappa_j X_l) (w_l - w_j)
+// (boundary neighbours: w_j = 0, kappa_j = 0).  Then kappa = psi / alpha and
+// the clamp psi = max(psi_pre, 0) (sph.py:219-226) give psi_bar (written),
+// alpha_bar and rho_bar (psi_pre = rho - rho_0 + ..., sph.py:196-199).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_adj_kappa(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
+                const T* __restrict__ fmass, const T* __restrict__ alpha,
+                const T* __restrict__ tkappa, const V4<T>* __restrict__ vbar,
+                const int* __restrict__ fstart, const V4<T>* __restrict__ bpos,
+                const int* __restrict__ bstart, const int* __restrict__ nbr_f,
+                const int* __restrict__ nbr_b, const int* __restrict__ nbr_nf,
+                const int* __restrict__ nbr_nb, T* __restrict__ psibar, T* __restrict__ rhobar,
+                T* __restrict__ alphabar, V4<T>* __restrict__ xbar, const TsphControl* ctrl) {
+  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const T dt = (T)ctrl->dt;
+  const V4<T> p = fpos[i];
+  const T Xi = p.w;
+  const T mi = fmass[i];
+  const V4<T> u = vbar[i];
+  const T wx = u.x / mi, wy = u.y / mi, wz = u.z / mi;
+  const T ki = tkappa[i];
+  T kb = 0, ax = 0, ay = 0, az = 0;
+  auto term = [&](const Rec2<T>& d) {
+    const V4<T>& q = d.a;  // q.w = X_j
+    const V4<T>& wj = d.b; // xyz = u_j / m_j, w = kappa_j
+    T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    D2<T> e = kern_d2(k, dx * dx + dy * dy + dz * dz);
+    T mx = wx - wj.x, my = wy - wj.y, mz = wz - wj.z;
+    kb += q.w * e.g * (dx * mx + dy * my + dz * mz);
+    T c = ki * q.w + wj.w * Xi;
+    add_J(ax, ay, az, e, dx, dy, dz, c * mx, c * my, c * mz);
+  };
+  sweep_lists<T, Rec2<T>>(
+      k, cap, i, p, fstart, bstart, fpos, bpos, nbr_f, nbr_b, nbr_nf[i], nbr_nb[i],
+      [&](int j) {
+        V4<T> uj = vbar[j];
+        T mj = fmass[j];
+        return Rec2<T>{fpos[j], v4(uj.x / mj, uj.y / mj, uj.z / mj, tkappa[j])};
+      },
+      [&](int j) { return Rec2<T>{bpos[j], v4(T(0), T(0), T(0), T(0))}; }, term, term);
+  const T s = -(T(1) / dt);
+  kb *= s;
+  const T al = alpha[i];
+  const T pb = ki > T(0) ? kb / al : T(0);
+  psibar[i] = pb;
+  rhobar[i] += pb;
+  alphabar[i] += -kb * ki / al;
+  V4<T> xb = xbar[i];
+  xbar[i] = v4(xb.x + s * ax, xb.y + s * ay, xb.z + s * az);
+}
+
+// Jacobi iteration reverse, half 2: IPPE_adv_psi (sph.py:202-216)
+//   psi_l = ... + dt sum_j X_j G(x_lj) . (v_l - v_j)
+// gives
+//   v_bar_l = u_l + dt sum_j (psib_l X_j + psib_j X_l) G(x_lj)
+//   x_bar_l += dt sum_j J (psib_l X_j + psib_j X_l) (v_l - v_j)
+// with v the recorded vel_adv entering the iteration (boundary: v_j = 0,
+// psib_j = 0).
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_adj_vel(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
+              const V4<T>* __restrict__ tvel, const T* __restrict__ psibar,
+              const V4<T>* __restrict__ vbar, const int* __restrict__ fstart,
+              const V4<T>* __restrict__ bpos, const int* __restrict__ bstart,
+              const int* __restrict__ nbr_f, const int* __restrict__ nbr_b,
+              const int* __restrict__ nbr_nf, const int* __restrict__ nbr_nb,
+              V4<T>* __restrict__ vbar_out, V4<T>* __restrict__ xbar, const TsphControl* ctrl) {
+  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const T dt = (T)ctrl->dt;
+  const V4<T> p = fpos[i];
+  const T Xi = p.w;
+  const V4<T> vi = tvel[i];
+  const T pbi = psibar[i];
+  T vx = 0, vy = 0, vz = 0, ax = 0, ay = 0, az = 0;
+  auto term = [&](const Rec2<T>& d) {
+    const V4<T>& q = d.a;   // q.w = X_j
+    const V4<T>& vj = d.b;  // xyz = vel_adv_j, w = psib_j
+    T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    D2<T> e = kern_d2(k, dx * dx + dy * dy + dz * dz);
+    T c = pbi * q.w + vj.w * Xi;
+    T cg = c * e.g;
+    vx += cg * dx;
+    vy += cg * dy;
+    vz += cg * dz;
+    add_J(ax, ay, az, e, dx, dy, dz, c * (vi.x - vj.x), c * (vi.y - vj.y), c * (vi.z - vj.z));
+  };
+  sweep_lists<T, Rec2<T>>(
+      k, cap, i, p, fstart, bstart, fpos, bpos, nbr_f, nbr_b, nbr_nf[i], nbr_nb[i],
+      [&](int j) {
+        V4<T> v = tvel[j];
+        v.w = psibar[j];
+        return Rec2<T>{fpos[j], v};
+      },
+      [&](int j) { return Rec2<T>{bpos[j], v4(T(0), T(0), T(0), T(0))}; }, term, term);
+  const V4<T> u = vbar[i];
+  vbar_out[i] = v4(u.x + dt * vx, u.y + dt * vy, u.z + dt * vz);
+  V4<T> xb = xbar[i];
+  xbar[i] = v4(xb.x + dt * ax, xb.y + dt * ay, xb.z + dt * az);
+}
+
+// ---------------------------------------------------------------------------
+// Advection reverse, part 1: surface tension loop 2 (sph.py:151-166)
+//   a_l += sum_j f_lj (-gamma m_j C(r) x/r + gamma (n_l - n_j)),
+//   f_lj = 2 rho0_l / (rho_l + rho_j),
+// over every candidate of the neighbouring cells with r > 0 (the curvature
+// term carries no kernel weight) and both volume fractions > 0.99.
+// With a_bar = dt u (vel_adv = vel + dt a, sph.py:247-250):
+//   n_bar_l  = gamma sum_j (f_lj a_l - f_jl a_j)
+//   x_bar_l += -gamma sum_j Jc (f_lj m_j a_l - f_jl m_l a_j),
+//              Jc w = (C/r) w + ((C' - C/r)/r^2)(x.w) x
+//   rho_bar_l += -sum_j (f_lj s_lj + f_jl s_jl) / (rho_l + rho_j)
+//   d loss / d gamma += a_l . sum_j f_lj (-m_j C x/r + n_l - n_j)
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_adj_surface_tension(KP<T> k, int n, const V4<T>* __restrict__ fpos,
+                          const T* __restrict__ fmass, const T* __restrict__ dens,
+                          const T* __restrict__ rdens, const T* __restrict__ vf,
+                          const int* __restrict__ fstart, const V4<T>* __restrict__ tnormal,
+                          const V4<T>* __restrict__ vbar, V4<T>* __restrict__ nbar,
+                          T* __restrict__ rhobar, V4<T>* __restrict__ xbar,
+                          double* __restrict__ partial, const TsphControl* ctrl) {
+  __shared__ double lds[kWaves];
+  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  double gsum = 0;
+  if (i < n) {
+    const T dt = (T)ctrl->dt;
+    const T gam = k.gamma;
+    T nbx = 0, nby = 0, nbz = 0, ax = 0, ay = 0, az = 0, rb = 0, gs = 0;
+    if (vf[(long)i * k.P] > T(0.99)) {
+      const V4<T> p = fpos[i];
+      const V4<T> ni = tnormal[i];
+      const V4<T> ui = vbar[i];
+      const T ali_x = dt * ui.x, ali_y = dt * ui.y, ali_z = dt * ui.z;
+      const T rhoi = dens[i], r0i = rdens[i], mi = fmass[i];
+      int c[3];
+      node_encode(k, p.x, p.y, p.z, c);
+      walk_cells(k, fstart, c, [&](int j) {
+        V4<T> q = fpos[j];
+        T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+        T r2 = dx * dx + dy * dy + dz * dz;
+        if (!(r2 > T(0)) || !(vf[(long)j * k.P] > T(0.99))) return;
+        T r = s_sqrt(r2);
+        T C, Cp;
+        coh_d1(k, r, &C, &Cp);
+        T cr = C / r;
+        T cq = (Cp - cr) / r2;
+        V4<T> nj = tnormal[j];
+        V4<T> uj = vbar[j];
+        T alj_x = dt * uj.x, alj_y = dt * uj.y, alj_z = dt * uj.z;
+        T rhoj = dens[j], r0j = rdens[j], mj = fmass[j];
+        T den = rhoi + rhoj;
+        T fij = T(2) * r0i / den, fji = T(2) * r0j / den;
+        T dnx = ni.x - nj.x, dny = ni.y - nj.y, dnz = ni.z - nj.z;
+        // gamma-free force of the pair on l
+        T tx = -mj * cr * dx + dnx, ty = -mj * cr * dy + dny, tz = -mj * cr * dz + dnz;
+        gs += fij * (ali_x * tx + ali_y * ty + ali_z * tz);
+        nbx += gam * (fij * ali_x - fji * alj_x);
+        nby += gam * (fij * ali_y - fji * alj_y);
+        nbz += gam * (fij * ali_z - fji * alj_z);
+        T wx = fij * mj * ali_x - fji * mi * alj_x;
+        T wy = fij * mj * ali_y - fji * mi * alj_y;
+        T wz = fij * mj * ali_z - fji * mi * alj_z;
+        T s = cq * (dx * wx + dy * wy + dz * wz);
+        ax += -gam * (cr * wx + s * dx);
+        ay += -gam * (cr * wy + s * dy);
+        az += -gam * (cr * wz + s * dz);
+        // s_jl: j's gamma-free force from l is m_l C x/r - (n_l - n_j)
+        T sl = ali_x * tx + ali_y * ty + ali_z * tz;
+        T sj = alj_x * (mi * cr * dx - dnx) + alj_y * (mi * cr * dy - dny) +
+               alj_z * (mi * cr * dz - dnz);
+        rb += -gam * (fij * sl + fji * sj) / den;
+      });
+    }
+    nbar[i] = v4(nbx, nby, nbz);
+    rhobar[i] += rb;
+    V4<T> xb = xbar[i];
+    xbar[i] = v4(xb.x + ax, xb.y + ay, xb.z + az);
+    gsum = (double)gs;
+  }
+  double s = block_sum(gsum, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// Advection reverse, part 2: viscosity (sph.py:114-129, W_lap sph_util.py:40-43)
+//   a_l += fv_l sum_j V_j w(r) (v_lj . x) x,  w = g / (0.01 h^2 + r^2),
+//   fv = 2 (2 + d) mu / rho0, V = X / rho,
+// and the normals (sph.py:141-149)  n_l = -h sum_j V_j G(x).  With a = dt u:
+//   v_bar_l  = u_l + sum_j w (e . x) x,           e = fv_l V_j a_l - fv_j V_l a_j
+//   x_bar_l += sum_j [w'/r (v.x)(e.x) x + w ((e.x) v + (v.x) e)]
+//              - h sum_j J (V_j nb_l - V_l nb_j)
+//   V_bar_l  = sum_j [-fv_j w (v.x)(a_j.x) + h g (nb_j . x)]
+//   rho_bar_l += -V_bar_l V_l / rho_l
+//   d loss / d mu += a_l . sum_j (fv_l / mu) V_j w (v.x) x
+template <class T>
+struct Rec4 {
+  V4<T> a, b, c, d;
+};
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_adj_viscosity(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
+                    const T* __restrict__ vol, const T* __restrict__ dens,
+                    const T* __restrict__ rdens, const int* __restrict__ fstart,
+                    const int* __restrict__ bstart, const V4<T>* __restrict__ bpos,
+                    const int* __restrict__ nbr_f, const int* __restrict__ nbr_nf,
+                    const V4<T>* __restrict__ tvel, const V4<T>* __restrict__ nbar,
+                    const V4<T>* __restrict__ vbar, V4<T>* __restrict__ vbar_out,
+                    T* __restrict__ rhobar, V4<T>* __restrict__ xbar,
+                    double* __restrict__ partial, const TsphControl* ctrl) {
+  __shared__ double lds[kWaves];
+  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  double msum = 0;
+  if (i < n) {
+    const T dt = (T)ctrl->dt;
+    const V4<T> p = fpos[i];
+    const V4<T> vi = tvel[i];
+    const V4<T> ui = vbar[i];
+    const V4<T> nbi = nbar[i];
+    const T ax_ = dt * ui.x, ay_ = dt * ui.y, az_ = dt * ui.z;
+    const T Vi = vol[i];
+    const T fmu_i = k.two_d2 / rdens[i];  // fv_i / mu
+    const T fvi = fmu_i * k.mu;
+    T vbx = 0, vby = 0, vbz = 0, xbx = 0, xby = 0, xbz = 0, Vb = 0, ms = 0;
+    auto term = [&](const Rec4<T>& d) {
+      const V4<T>& q = d.a;   // position
+      const V4<T>& vj = d.b;  // velocity, w = V_j
+      const V4<T>& aj = d.c;  // a_bar_j, w = fv_j
+      const V4<T>& nj = d.d;  // n_bar_j
+      T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+      T r2 = dx * dx + dy * dy + dz * dz;
+      D2<T> e = kern_d2(k, r2);
+      T den = k.eps + r2;
+      T rden = T(1) / den;
+      T w = e.g * rden;
+      T wq = (e.hq - T(2) * w) * rden;  // w'(r) / r
+      T Vj = vj.w, fvj = aj.w;
+      T vdx = vi.x - vj.x, vdy = vi.y - vj.y, vdz = vi.z - vj.z;
+      T vx = vdx * dx + vdy * dy + vdz * dz;
+      T alx = ax_ * dx + ay_ * dy + az_ * dz;
+      ms += fmu_i * Vj * w * vx * alx;
+      T ex_ = fvi * Vj * ax_ - fvj * Vi * aj.x;
+      T ey_ = fvi * Vj * ay_ - fvj * Vi * aj.y;
+      T ez_ = fvi * Vj * az_ - fvj * Vi * aj.z;
+      T ex = ex_ * dx + ey_ * dy + ez_ * dz;
+      vbx += w * ex * dx;
+      vby += w * ex * dy;
+      vbz += w * ex * dz;
+      T t = wq * vx * ex;
+      xbx += t * dx + w * (ex * vdx + vx * ex_);
+      xby += t * dy + w * (ex * vdy + vx * ey_);
+      xbz += t * dz + w * (ex * vdz + vx * ez_);
+      T ajx = aj.x * dx + aj.y * dy + aj.z * dz;
+      Vb += -fvj * w * vx * ajx + k.h * e.g * (nj.x * dx + nj.y * dy + nj.z * dz);
+      T cx = Vj * nbi.x - Vi * nj.x, cy = Vj * nbi.y - Vi * nj.y, cz = Vj * nbi.z - Vi * nj.z;
+      T s = e.hq * (dx * cx + dy * cy + dz * cz);
+      xbx += -k.h * (e.g * cx + s * dx);
+      xby += -k.h * (e.g * cy + s * dy);
+      xbz += -k.h * (e.g * cz + s * dz);
+    };
+    sweep_lists<T, Rec4<T>>(
+        k, cap, i, p, fstart, bstart, fpos, bpos, nbr_f, nullptr, nbr_nf[i], 0,
+        [&](int j) {
+          V4<T> v = tvel[j];
+          v.w = vol[j];
+          V4<T> u = vbar[j];
+          V4<T> a = v4(dt * u.x, dt * u.y, dt * u.z, k.two_d2 * k.mu / rdens[j]);
+          return Rec4<T>{fpos[j], v, a, nbar[j]};
+        },
+        [&](int j) { return Rec4<T>{}; }, term, [&](const Rec4<T>&) {});
+    vbar_out[i] = v4(ui.x + vbx, ui.y + vby, ui.z + vbz);
+    rhobar[i] += -Vb * Vi / dens[i];
+    V4<T> xb = xbar[i];
+    xbar[i] = v4(xb.x + xbx, xb.y + xby, xb.z + xbz);
+    msum = (double)ms;
+  }
+  double s = block_sum(msum, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// Adds the block partials of a parameter gradient into grad[slot], in a
+// fixed order.
+__global__ void __launch_bounds__(kBlock)
+    k_adj_grad_add(const double* __restrict__ partial, int nblk, double* grad, int slot) {
+  __shared__ double lds[kWaves];
+  double a = sum_partials(partial, nblk, lds);
+  if (threadIdx.x == 0) grad[slot] += a;
+}
+
+// ---------------------------------------------------------------------------
+// Pass A reverse.  alpha = max(|alpha_1|^2 / m + alpha_2, 1e-4) (sph.py:100-105):
+// the clamp passes no gradient; a1bar = (2 alpha_bar alpha_1 / m, alpha_bar).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_adj_alpha_prep(int n, const T* __restrict__ alpha, const V4<T>* __restrict__ alpha_1,
+                     const T* __restrict__ fmass, const T* __restrict__ alphabar,
+                     V4<T>* __restrict__ a1bar) {
+  int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  T ab = alpha[i] > T(1e-4) ? alphabar[i] : T(0);
+  V4<T> a1 = alpha_1[i];
+  T s = T(2) * ab / fmass[i];
+  a1bar[i] = v4(s * a1.x, s * a1.y, s * a1.z, ab);
+}
+
+// rho_l = sum_j m_j W(r)            -> x_bar_l += (rb_l m_j + rb_j m_l) g x
+// alpha_1_l = sum_j X_j G(x)         -> x_bar_l += J (X_j a1b_l - X_l a1b_j)
+// alpha_2_l = sum_j W'^2 X_j^2 / m_j -> x_bar_l += 2 g W'' (a2b_l X_j^2/m_j + a2b_j X_l^2/m_l) x
+// (boundary neighbours: density and alpha_1 terms of l's equation only;
+// alpha_2 of the fluid is (f,f) only, sph.py:194).
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_adj_pass_a(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
+                 const T* __restrict__ fmass, const T* __restrict__ rhobar,
+                 const V4<T>* __restrict__ a1bar, const int* __restrict__ fstart,
+                 const V4<T>* __restrict__ bpos, const T* __restrict__ bmass,
+                 const int* __restrict__ bstart, const int* __restrict__ nbr_f,
+                 const int* __restrict__ nbr_b, const int* __restrict__ nbr_nf,
+                 const int* __restrict__ nbr_nb, V4<T>* __restrict__ xbar) {
+  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const V4<T> p = fpos[i];
+  const T Xi = p.w, mi = fmass[i];
+  const T rbi = rhobar[i];
+  const V4<T> abi = a1bar[i];
+  const T xm_i = Xi * Xi / mi;
+  T ax = 0, ay = 0, az = 0;
+  // record: a = position (w = X_j), b = a1bar_j (w = a2bar_j),
+  // c = (rho_bar_j, m_j, is_fluid)
+  sweep_lists<T, Rec4<T>>(
+      k, cap, i, p, fstart, bstart, fpos, bpos, nbr_f, nbr_b, nbr_nf[i], nbr_nb[i],
+      [&](int j) {
+        V4<T> extra = v4(rhobar[j], fmass[j], T(1), T(0));
+        return Rec4<T>{fpos[j], a1bar[j], extra, V4<T>{}};
+      },
+      [&](int j) {
+        V4<T> extra = v4(T(0), bmass[j], T(0), T(0));
+        return Rec4<T>{bpos[j], V4<T>{}, extra, V4<T>{}};
+      },
+      [&](const Rec4<T>& d) {
+        const V4<T>& q = d.a;
+        const V4<T>& abj = d.b;
+        const T rbj = d.c.x, mj = d.c.y;
+        const bool fluid = d.c.z != T(0);
+        T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+        D2<T> e = kern_d2(k, dx * dx + dy * dy + dz * dz);
+        T cd = (rbi * mj + rbj * mi) * e.g;
+        T c2 = fluid ? T(2) * e.g * e.w2 * (abi.w * q.w * q.w / mj + abj.w * xm_i) : T(0);
+        T s1 = cd + c2;
+        ax += s1 * dx;
+        ay += s1 * dy;
+        az += s1 * dz;
+        add_J(ax, ay, az, e, dx, dy, dz, q.w * abi.x - Xi * abj.x, q.w * abi.y - Xi * abj.y,
+              q.w * abi.z - Xi * abj.z);
+      },
+      [&](const Rec4<T>& d) {
+        const V4<T>& q = d.a;
+        const T mj = d.c.y;
+        T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+        D2<T> e = kern_d2(k, dx * dx + dy * dy + dz * dz);
+        T cd = rbi * mj * e.g;
+        ax += cd * dx;
+        ay += cd * dy;
+        az += cd * dz;
+        add_J(ax, ay, az, e, dx, dy, dz, q.w * abi.x, q.w * abi.y, q.w * abi.z);
+      });
+  V4<T> xb = xbar[i];
+  xbar[i] = v4(xb.x + ax, xb.y + ay, xb.z + az);
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+// ---------------------------------------------------------------------------
+template <class T>
+int adj_begin(const TsphParams& prm, TsphObject* f, TsphControl* ctrl, TsphAdjoint* a,
+              const int* pid, const void* xref, const void* vref, hipStream_t st) {
+  (void)prm;
+  if (f->n == 0) return 0;
+  hipLaunchKernelGGL(k_adj_begin<T>, dim3(blocks_for(f->n)), dim3(kBlock), 0, st, f->n, pid,
+                     (const V4<T>*)xref, (const V4<T>*)vref, (V4<T>*)a->xbar, (V4<T>*)a->vbar,
+                     (T*)a->rhobar, (T*)a->alphabar, ctrl);
+  TSPH_LAUNCH_CHECK("adj_begin");
+  return 0;
+}
+
+template <class T>
+int adj_end(const TsphParams& prm, TsphObject* f, TsphAdjoint* a, const int* pid, void* xref,
+            void* vref, hipStream_t st) {
+  (void)prm;
+  if (f->n == 0) return 0;
+  hipLaunchKernelGGL(k_adj_end<T>, dim3(blocks_for(f->n)), dim3(kBlock), 0, st, f->n, pid,
+                     (const V4<T>*)a->xbar, (const V4<T>*)a->vbar, (V4<T>*)xref, (V4<T>*)vref);
+  TSPH_LAUNCH_CHECK("adj_end");
+  return 0;
+}
+
+template <class T>
+int adj_solver_iteration(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl* ctrl,
+                         TsphAdjoint* a, void* ws, size_t wsb, hipStream_t st) {
+  (void)ws;
+  (void)wsb;
+  KP<T> k = make_kp<T>(prm);
+  if (f->n == 0) return 0;
+  int nb = blocks_for(f->n);
+  hipLaunchKernelGGL(k_adj_kappa<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
+                     (const V4<T>*)f->pos, (const T*)f->mass, (const T*)f->alpha,
+                     (const T*)a->tape_kappa, (const V4<T>*)a->vbar, f->cell_start,
+                     (const V4<T>*)b->pos, b->cell_start, f->nbr_f, f->nbr_b, f->nbr_nf,
+                     f->nbr_nb, (T*)a->psibar, (T*)a->rhobar, (T*)a->alphabar, (V4<T>*)a->xbar,
+                     ctrl);
+  hipLaunchKernelGGL(k_adj_vel<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
+                     (const V4<T>*)f->pos, (const V4<T>*)a->tape_vel_adv, (const T*)a->psibar,
+                     (const V4<T>*)a->vbar, f->cell_start, (const V4<T>*)b->pos, b->cell_start,
+                     f->nbr_f, f->nbr_b, f->nbr_nf, f->nbr_nb, (V4<T>*)a->vbar_out,
+                     (V4<T>*)a->xbar, ctrl);
+  TSPH_LAUNCH_CHECK("adj_solver_iteration");
+  return 0;
+}
+
+template <class T>
+int adj_advect(const TsphParams& prm, TsphObject* f, TsphControl* ctrl, TsphAdjoint* a, void* ws,
+               size_t wsb, hipStream_t st) {
+  KP<T> k = make_kp<T>(prm);
+  WsLayout L;
+  TSPH_REQUIRE(carve_ws(ws, wsb, f->capacity, prm.node_num, &L), TSPH_ECAPACITY,
+               "tsph_adj_advect: workspace too small");
+  if (f->n == 0) return 0;
+  int nb = blocks_for(f->n);
+  hipLaunchKernelGGL(k_adj_surface_tension<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n,
+                     (const V4<T>*)f->pos, (const T*)f->mass, (const T*)f->sph_density,
+                     (const T*)f->rest_density, (const T*)f->volume_frac, f->cell_start,
+                     (const V4<T>*)a->tape_normal, (const V4<T>*)a->vbar, (V4<T>*)a->nbar,
+                     (T*)a->rhobar, (V4<T>*)a->xbar, L.partial, ctrl);
+  hipLaunchKernelGGL(k_adj_grad_add, dim3(1), dim3(kBlock), 0, st, L.partial, nb, a->grad, 1);
+  hipLaunchKernelGGL(k_adj_viscosity<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
+                     (const V4<T>*)f->pos, (const T*)f->vol, (const T*)f->sph_density,
+                     (const T*)f->rest_density, f->cell_start, (const int*)nullptr,
+                     (const V4<T>*)nullptr, f->nbr_f, f->nbr_nf, (const V4<T>*)a->tape_vel,
+                     (const V4<T>*)a->nbar, (const V4<T>*)a->vbar, (V4<T>*)a->vbar_out,
+                     (T*)a->rhobar, (V4<T>*)a->xbar, L.partial, ctrl);
+  hipLaunchKernelGGL(k_adj_grad_add, dim3(1), dim3(kBlock), 0, st, L.partial, nb, a->grad, 0);
+  TSPH_LAUNCH_CHECK("adj_advect");
+  return 0;
+}
+
+template <class T>
+int adj_density_alpha(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphAdjoint* a,
+                      hipStream_t st) {
+  KP<T> k = make_kp<T>(prm);
+  if (f->n == 0) return 0;
+  int nb = blocks_for(f->n);
+  hipLaunchKernelGGL(k_adj_alpha_prep<T>, dim3(nb), dim3(kBlock), 0, st, f->n, (const T*)f->alpha,
+                     (const V4<T>*)f->alpha_1, (const T*)f->mass, (const T*)a->alphabar,
+                     (V4<T>*)a->a1bar);
+  hipLaunchKernelGGL(k_adj_pass_a<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
+                     (const V4<T>*)f->pos, (const T*)f->mass, (const T*)a->rhobar,
+                     (const V4<T>*)a->a1bar, f->cell_start, (const V4<T>*)b->pos,
+                     (const T*)b->mass, b->cell_start, f->nbr_f, f->nbr_b, f->nbr_nf, f->nbr_nb,
+                     (V4<T>*)a->xbar);
+  TSPH_LAUNCH_CHECK("adj_density_alpha");
+  return 0;
+}
+
+#define TSPH_INST(T)                                                                           \
+  template int adj_begin<T>(const TsphParams&, TsphObject*, TsphControl*, TsphAdjoint*,         \
+                            const int*, const void*, const void*, hipStream_t);                 \
+  template int adj_end<T>(const TsphParams&, TsphObject*, TsphAdjoint*, const int*, void*,      \
+                          void*, hipStream_t);                                                  \
+  template int adj_solver_iteration<T>(const TsphParams&, TsphObject*, TsphObject*,             \
+                                       TsphControl*, TsphAdjoint*, void*, size_t, hipStream_t); \
+  template int adj_advect<T>(const TsphParams&, TsphObject*, TsphControl*, TsphAdjoint*, void*, \
+                             size_t, hipStream_t);                                              \
+  template int adj_density_alpha<T>(const TsphParams&, TsphObject*, TsphObject*, TsphAdjoint*,  \
+                                    hipStream_t);
+TSPH_INST(float)
+TSPH_INST(double)
+#undef TSPH_INST
+
+}  // namespace tsph

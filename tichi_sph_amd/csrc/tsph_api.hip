@@ -27,8 +27,23 @@ int density_alpha(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, voi
 template <class T>
 int solver_begin(const TsphParams&, TsphControl*, int, hipStream_t);
 template <class T>
-int solver_half(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, int, void*, size_t,
-                hipStream_t);
+int solver_half(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, int, void*, void*,
+                void*, size_t, hipStream_t);
+template <class T>
+int relist(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
+template <class T>
+int adj_begin(const TsphParams&, TsphObject*, TsphControl*, TsphAdjoint*, const int*, const void*,
+              const void*, hipStream_t);
+template <class T>
+int adj_end(const TsphParams&, TsphObject*, TsphAdjoint*, const int*, void*, void*, hipStream_t);
+template <class T>
+int adj_solver_iteration(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, TsphAdjoint*,
+                         void*, size_t, hipStream_t);
+template <class T>
+int adj_advect(const TsphParams&, TsphObject*, TsphControl*, TsphAdjoint*, void*, size_t,
+               hipStream_t);
+template <class T>
+int adj_density_alpha(const TsphParams&, TsphObject*, TsphObject*, TsphAdjoint*, hipStream_t);
 template <class T>
 int advect(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template <class T>
@@ -100,6 +115,14 @@ static int check_params(const TsphParams* p) {
   return 0;
 }
 
+// The adjoint differentiates one-phase DFSPH only.
+static int check_adj(const TsphParams* p, const TsphAdjoint* a, const char* what) {
+  TSPH_REQUIRE(a, TSPH_EINVAL, "%s: null adjoint", what);
+  TSPH_REQUIRE(p->solver_type == TSPH_DFSPH && p->phase_num == 1, TSPH_EINVAL,
+               "%s: the adjoint supports one-phase DFSPH only", what);
+  return 0;
+}
+
 static int check_obj(const TsphObject* o, const char* what) {
   TSPH_REQUIRE(o, TSPH_EINVAL, "%s: null object", what);
   TSPH_REQUIRE(o->n >= 0 && o->n <= o->capacity, TSPH_ECAPACITY,
@@ -167,13 +190,15 @@ int tsph_solver_begin(const TsphParams* p, TsphControl* ctrl, int32_t solve, voi
 }
 
 static int solver_entry(const TsphParams* p, TsphObject* f, TsphObject* b, TsphControl* ctrl,
-                        int32_t k, int half, void* ws, size_t wsb, void* stream) {
+                        int32_t k, int half, void* ws, size_t wsb, void* stream,
+                        void* tape_v = nullptr, void* tape_k = nullptr) {
   if (int rc = check_params(p)) return rc;
   if (int rc = check_obj(f, "tsph_solver_iteration")) return rc;
   if (int rc = check_obj(b, "tsph_solver_iteration")) return rc;
   TSPH_REQUIRE(k >= 0 && k < TSPH_MAX_ITER_LOG, TSPH_EINVAL,
                "tsph_solver_iteration: iteration %d outside [0, %d)", k, TSPH_MAX_ITER_LOG);
-  return DISPATCH(p, solver_half, *p, f, b, ctrl, k, half, ws, wsb, (hipStream_t)stream);
+  return DISPATCH(p, solver_half, *p, f, b, ctrl, k, half, tape_v, tape_k, ws, wsb,
+                  (hipStream_t)stream);
 }
 
 int tsph_solver_iteration(const TsphParams* p, TsphObject* f, TsphObject* b, TsphControl* ctrl,
@@ -187,6 +212,22 @@ int tsph_solver_psi(const TsphParams* p, TsphObject* f, TsphObject* b, TsphContr
 int tsph_solver_update(const TsphParams* p, TsphObject* f, TsphObject* b, TsphControl* ctrl,
                        int32_t k, void* ws, size_t wsb, void* stream) {
   return solver_entry(p, f, b, ctrl, k, 2, ws, wsb, stream);
+}
+int tsph_solver_psi_taped(const TsphParams* p, TsphObject* f, TsphObject* b, TsphControl* ctrl,
+                          int32_t k, void* tape_vel_adv, void* tape_kappa, void* ws, size_t wsb,
+                          void* stream) {
+  TSPH_REQUIRE(tape_vel_adv && tape_kappa, TSPH_EINVAL, "tsph_solver_psi_taped: null tape");
+  return solver_entry(p, f, b, ctrl, k, 1, ws, wsb, stream, tape_vel_adv, tape_kappa);
+}
+
+int tsph_relist(const TsphParams* p, TsphObject* f, TsphObject* b, TsphControl* ctrl, void* ws,
+                size_t wsb, void* stream) {
+  if (int rc = check_params(p)) return rc;
+  if (int rc = check_obj(f, "tsph_relist")) return rc;
+  if (int rc = check_obj(b, "tsph_relist")) return rc;
+  TSPH_REQUIRE(p->list_cap_f >= 1 && p->list_cap_b >= 1, TSPH_EINVAL,
+               "tsph_relist: list capacities must be positive");
+  return DISPATCH(p, relist, *p, f, b, ctrl, ws, wsb, (hipStream_t)stream);
 }
 
 int tsph_advect(const TsphParams* p, TsphObject* f, TsphObject* b, TsphControl* ctrl, void* ws,
@@ -208,6 +249,55 @@ int tsph_tail(const TsphParams* p, TsphObject* f, TsphControl* ctrl, int32_t fbm
   if (int rc = check_params(p)) return rc;
   if (int rc = check_obj(f, "tsph_tail")) return rc;
   return DISPATCH(p, tail, *p, f, ctrl, fbm, ws, wsb, (hipStream_t)stream);
+}
+
+int tsph_adj_begin(const TsphParams* p, TsphObject* f, TsphControl* ctrl, TsphAdjoint* adj,
+                   const int32_t* pid, const void* xbar_ref, const void* vbar_ref, void* stream) {
+  if (int rc = check_params(p)) return rc;
+  if (int rc = check_obj(f, "tsph_adj_begin")) return rc;
+  if (int rc = check_adj(p, adj, "tsph_adj_begin")) return rc;
+  TSPH_REQUIRE(pid && xbar_ref && vbar_ref, TSPH_EINVAL, "tsph_adj_begin: null array");
+  return DISPATCH(p, adj_begin, *p, f, ctrl, adj, pid, xbar_ref, vbar_ref, (hipStream_t)stream);
+}
+
+int tsph_adj_end(const TsphParams* p, TsphObject* f, TsphAdjoint* adj, const int32_t* pid,
+                 void* xbar_ref, void* vbar_ref, void* stream) {
+  if (int rc = check_params(p)) return rc;
+  if (int rc = check_obj(f, "tsph_adj_end")) return rc;
+  if (int rc = check_adj(p, adj, "tsph_adj_end")) return rc;
+  TSPH_REQUIRE(pid && xbar_ref && vbar_ref, TSPH_EINVAL, "tsph_adj_end: null array");
+  return DISPATCH(p, adj_end, *p, f, adj, pid, xbar_ref, vbar_ref, (hipStream_t)stream);
+}
+
+int tsph_adj_solver_iteration(const TsphParams* p, TsphObject* f, TsphObject* b,
+                              TsphControl* ctrl, TsphAdjoint* adj, void* ws, size_t wsb,
+                              void* stream) {
+  if (int rc = check_params(p)) return rc;
+  if (int rc = check_obj(f, "tsph_adj_solver_iteration")) return rc;
+  if (int rc = check_obj(b, "tsph_adj_solver_iteration")) return rc;
+  if (int rc = check_adj(p, adj, "tsph_adj_solver_iteration")) return rc;
+  TSPH_REQUIRE(adj->tape_vel_adv && adj->tape_kappa, TSPH_EINVAL,
+               "tsph_adj_solver_iteration: null tape");
+  return DISPATCH(p, adj_solver_iteration, *p, f, b, ctrl, adj, ws, wsb, (hipStream_t)stream);
+}
+
+int tsph_adj_advect(const TsphParams* p, TsphObject* f, TsphControl* ctrl, TsphAdjoint* adj,
+                    void* ws, size_t wsb, void* stream) {
+  if (int rc = check_params(p)) return rc;
+  if (int rc = check_obj(f, "tsph_adj_advect")) return rc;
+  if (int rc = check_adj(p, adj, "tsph_adj_advect")) return rc;
+  TSPH_REQUIRE(adj->tape_vel && adj->tape_normal && adj->grad, TSPH_EINVAL,
+               "tsph_adj_advect: null tape or grad");
+  return DISPATCH(p, adj_advect, *p, f, ctrl, adj, ws, wsb, (hipStream_t)stream);
+}
+
+int tsph_adj_density_alpha(const TsphParams* p, TsphObject* f, TsphObject* b, TsphAdjoint* adj,
+                           void* stream) {
+  if (int rc = check_params(p)) return rc;
+  if (int rc = check_obj(f, "tsph_adj_density_alpha")) return rc;
+  if (int rc = check_obj(b, "tsph_adj_density_alpha")) return rc;
+  if (int rc = check_adj(p, adj, "tsph_adj_density_alpha")) return rc;
+  return DISPATCH(p, adj_density_alpha, *p, f, b, adj, (hipStream_t)stream);
 }
 
 int tsph_ref_kernel(const TsphParams* p, int32_t kernel, TsphObject* obj, const TsphObject* nobj,

@@ -9,35 +9,9 @@
 // (0 < r < h, exact f32 test) of each fluid particle in reference order;
 // every later sweep iterates only those lists.  Terms beyond h are exact
 // zeros in the reference, so skipping them changes no value.
-#include "tsph_host.h"
+#include "tsph_sweep.h"
 
 namespace tsph {
-
-// Visit every candidate slot of `cstart`'s object in the cells around c[].
-template <class T, class F>
-__device__ __forceinline__ void walk_cells(const KP<T>& k, const int* __restrict__ cstart,
-                                           const int* c, F&& f) {
-  const int R = k.R;
-  const int zr = k.dim == 3 ? R : 0;
-#pragma unroll 1
-  for (int dz = -zr; dz <= zr; ++dz) {
-#pragma unroll 1
-    for (int dy = -R; dy <= R; ++dy) {
-      int c0 = dim_encode(k, wadd(c[0], -R), wadd(c[1], dy), wadd(c[2], dz));
-      long lo = c0 < 1 ? 1 : (long)c0;
-      long hi = (long)c0 + 2 * R;
-      if (hi > k.node_num - 1) hi = k.node_num - 1;
-      if (lo > hi) continue;
-      int j0 = cstart[lo], j1 = cstart[hi + 1];
-      for (int j = j0; j < j1; ++j) f(j);
-    }
-  }
-}
-
-template <class T>
-__device__ __forceinline__ T rest_psi_of(const KP<T>& k, const T* rdens, int i) {
-  return k.solver == TSPH_VFSPH ? T(1) : rdens[i];
-}
 
 // Pair terms of pass A for one in-range candidate (strict arithmetic, the
 // order of sph.py:61-64, 81, 97).
@@ -234,107 +208,6 @@ __global__ void k_solver_begin(KP<T> k, TsphControl* ctrl, int solve) {
   }
 }
 
-// Iterate a particle's fluid neighbours: the recorded list, or, when the list
-// overflowed, the cell walk with the same in-range test.
-template <class T, class L, class F>
-__device__ __forceinline__ void for_fluid_nbrs(const KP<T>& k, int i, int cap, int nf,
-                                               const L& list,
-                                               const int* __restrict__ cstart,
-                                               const V4<T>* __restrict__ pos, V4<T> p, F&& f) {
-  if (nf <= k.list_cap_f) {
-    for (int s = 0; s < nf; ++s) f(list[(long)s * cap + i]);
-  } else {
-    int c[3];
-    node_encode(k, p.x, p.y, p.z, c);
-    walk_cells(k, cstart, c, [&](int j) {
-      V4<T> q = pos[j];
-      T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
-      if (r2 > T(0) && r2 <= k.r2max) f(j);
-    });
-  }
-}
-template <class T, class F>
-__device__ __forceinline__ void for_bound_nbrs(const KP<T>& k, int i, int cap, int nbb,
-                                               const int* __restrict__ list,
-                                               const int* __restrict__ cstart,
-                                               const V4<T>* __restrict__ bpos, V4<T> p, F&& f) {
-  if (nbb <= k.list_cap_b) {
-    for (int s = 0; s < nbb; ++s) f(list[(long)s * cap + i]);
-  } else {
-    int c[3];
-    node_encode(k, p.x, p.y, p.z, c);
-    walk_cells(k, cstart, c, [&](int j) {
-      V4<T> q = bpos[j];
-      T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
-      if (r2 > T(0) && r2 <= k.r2max) f(j);
-    });
-  }
-}
-
-// Neighbour iteration of the list sweeps for one lane, in the recorded
-// (reference) order: fluid list, then boundary list.  The sweeps are bound
-// by the latency of dependent loads (list entry -> neighbour data), so the
-// lists are consumed in batches of kNb: all kNb indices are loaded, then all
-// their neighbour records, then the pair terms are applied in order.  A lane
-// whose list overflowed walks the cells with the same in-range test.
-constexpr int kNb = 8;
-
-template <class T, class Rec, class LoadF, class LoadB, class PairF, class PairB>
-__device__ __forceinline__ void sweep_lists(const KP<T>& k, int cap, int i, V4<T> p,
-                                            const int* __restrict__ fstart,
-                                            const int* __restrict__ bstart,
-                                            const V4<T>* __restrict__ fpos,
-                                            const V4<T>* __restrict__ bpos,
-                                            const int* __restrict__ nbr_f,
-                                            const int* __restrict__ nbr_b, int nf, int nbb,
-                                            LoadF&& load_f, LoadB&& load_b, PairF&& pair_f,
-                                            PairB&& pair_b) {
-  if (nf <= k.list_cap_f && nbb <= k.list_cap_b) {
-    for (int s0 = 0; s0 < nf; s0 += kNb) {
-      int j[kNb];
-      Rec d[kNb];
-#pragma unroll
-      for (int u = 0; u < kNb; ++u) j[u] = s0 + u < nf ? nbr_f[(long)(s0 + u) * cap + i] : -1;
-#pragma unroll
-      for (int u = 0; u < kNb; ++u)
-        if (j[u] >= 0) d[u] = load_f(j[u]);
-#pragma unroll
-      for (int u = 0; u < kNb; ++u)
-        if (j[u] >= 0) pair_f(d[u]);
-    }
-    for (int s0 = 0; s0 < nbb; s0 += kNb) {
-      int j[kNb];
-      Rec d[kNb];
-#pragma unroll
-      for (int u = 0; u < kNb; ++u) j[u] = s0 + u < nbb ? nbr_b[(long)(s0 + u) * cap + i] : -1;
-#pragma unroll
-      for (int u = 0; u < kNb; ++u)
-        if (j[u] >= 0) d[u] = load_b(j[u]);
-#pragma unroll
-      for (int u = 0; u < kNb; ++u)
-        if (j[u] >= 0) pair_b(d[u]);
-    }
-    return;
-  }
-  int c[3];
-  node_encode(k, p.x, p.y, p.z, c);
-  walk_cells(k, fstart, c, [&](int j) {
-    V4<T> q = fpos[j];
-    T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
-    if (r2 > T(0) && r2 <= k.r2max) pair_f(load_f(j));
-  });
-  walk_cells(k, bstart, c, [&](int j) {
-    V4<T> q = bpos[j];
-    T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
-    if (r2 > T(0) && r2 <= k.r2max) pair_b(load_b(j));
-  });
-}
-
-template <class T>
-struct Rec2 {
-  V4<T> a, b;
-};
-
 // Jacobi half-step P: IPPE_adv_psi_init + IPPE_adv_psi (f,f)(f,b) +
 // IPPE_psi_adv_non_negative (sph.py:196-226) for iteration kk of the solve.
 template <class T>
@@ -346,7 +219,8 @@ __global__ void __launch_bounds__(kBlock)
                  const int* __restrict__ bstart, const int* __restrict__ nbr_f,
                  const int* __restrict__ nbr_b, const int* __restrict__ nbr_nf,
                  const int* __restrict__ nbr_nb, T* __restrict__ psi_adv, T* __restrict__ kappa,
-                 double* __restrict__ partial, TsphControl* ctrl) {
+                 V4<T>* __restrict__ tape_v, T* __restrict__ tape_k, double* __restrict__ partial,
+                 TsphControl* ctrl) {
   __shared__ double lds[kWaves];
   if (!ctrl->run_u[kk]) return;
   const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
@@ -373,7 +247,12 @@ __global__ void __launch_bounds__(kBlock)
         [&](int j) { return Rec2<T>{bpos[j], V4<T>{T(0), T(0), T(0), T(0)}}; }, term, term);
     if (psi < T(0)) psi = T(0);
     psi_adv[i] = psi;
-    kappa[i] = psi / alpha[i];
+    const T kap = psi / alpha[i];
+    kappa[i] = kap;
+    if (tape_v) {  // record the iteration for the adjoint
+      tape_v[i] = va;
+      tape_k[i] = kap;
+    }
     err = (double)(psi / rp);
   }
   double s = block_sum(err, lds);
@@ -730,12 +609,9 @@ __global__ void __launch_bounds__(kBlock)
 // Host launchers.
 // ---------------------------------------------------------------------------
 template <class T>
-int density_alpha(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl* ctrl,
-                  void* ws, size_t wsb, hipStream_t st) {
+static int pass_a(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl* ctrl,
+                  bool with_bound, hipStream_t st) {
   KP<T> k = make_kp<T>(prm);
-  WsLayout L;
-  TSPH_REQUIRE(carve_ws(ws, wsb, f->capacity, prm.node_num, &L), TSPH_ECAPACITY,
-               "tsph_density_alpha: workspace too small");
   hipLaunchKernelGGL(k_step_reset, dim3(1), dim3(64), 0, st, ctrl);
   if (f->n > 0)
     hipLaunchKernelGGL(k_pass_a_fluid<T>, dim3(blocks_for(f->n)), dim3(kBlock), 0, st, k, f->n,
@@ -746,7 +622,7 @@ int density_alpha(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphContr
                        (T*)f->alpha_2, (T*)f->alpha, (T*)f->vol, (const V4<T>*)f->vel,
                        (V4<T>*)f->vel_adv, f->nbr_f, f->nbr_b, f->nbr_nf, f->nbr_nb, f->flag,
                        ctrl);
-  if (b->n > 0)
+  if (with_bound && b->n > 0)
     hipLaunchKernelGGL(k_pass_a_bound<T>, dim3(blocks_for(b->n)), dim3(kBlock), 0, st, k, b->n,
                        (const V4<T>*)b->pos, (const T*)b->mass, (const T*)b->rest_volume,
                        b->cell_start, (const V4<T>*)f->pos, (const T*)f->mass,
@@ -755,6 +631,24 @@ int density_alpha(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphContr
                        (T*)b->vol);
   TSPH_LAUNCH_CHECK("density_alpha");
   return 0;
+}
+
+template <class T>
+int density_alpha(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl* ctrl,
+                  void* ws, size_t wsb, hipStream_t st) {
+  WsLayout L;
+  TSPH_REQUIRE(carve_ws(ws, wsb, f->capacity, prm.node_num, &L), TSPH_ECAPACITY,
+               "tsph_density_alpha: workspace too small");
+  return pass_a<T>(prm, f, b, ctrl, true, st);
+}
+
+// The fluid half of pass A alone: rebuilds the neighbour lists and the
+// fluid's density / alpha fields from recorded sorted positions.  The adjoint
+// recomputes per-step lists this way instead of storing them.
+template <class T>
+int relist(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl* ctrl, void* ws,
+           size_t wsb, hipStream_t st) {
+  return pass_a<T>(prm, f, b, ctrl, false, st);
 }
 
 template <class T>
@@ -767,7 +661,7 @@ int solver_begin(const TsphParams& prm, TsphControl* ctrl, int solve, hipStream_
 
 template <class T>
 int solver_half(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl* ctrl, int kk,
-                int half, void* ws, size_t wsb, hipStream_t st) {
+                int half, void* tape_v, void* tape_k, void* ws, size_t wsb, hipStream_t st) {
   KP<T> k = make_kp<T>(prm);
   WsLayout L;
   TSPH_REQUIRE(carve_ws(ws, wsb, f->capacity, prm.node_num, &L), TSPH_ECAPACITY,
@@ -780,7 +674,8 @@ int solver_half(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl
                        (const V4<T>*)f->pos, (const V4<T>*)f->vel_adv, psi,
                        (const T*)f->rest_density, (const T*)f->alpha, f->cell_start,
                        (const V4<T>*)b->pos, b->cell_start, f->nbr_f, f->nbr_b, f->nbr_nf,
-                       f->nbr_nb, (T*)f->psi_adv, (T*)f->kappa, L.partial, ctrl);
+                       f->nbr_nb, (T*)f->psi_adv, (T*)f->kappa, (V4<T>*)tape_v, (T*)tape_k,
+                       L.partial, ctrl);
   if (half & 2)
     hipLaunchKernelGGL(k_solver_update<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
                        kk, (const V4<T>*)f->pos, (const T*)f->mass, (const T*)f->kappa,
@@ -856,12 +751,14 @@ int tail(const TsphParams& prm, TsphObject* f, TsphControl* ctrl, int fbm, void*
   return 0;
 }
 
+template int relist<float>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
+template int relist<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template int density_alpha<float>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template int density_alpha<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template int solver_begin<float>(const TsphParams&, TsphControl*, int, hipStream_t);
 template int solver_begin<double>(const TsphParams&, TsphControl*, int, hipStream_t);
-template int solver_half<float>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, int, void*, size_t, hipStream_t);
-template int solver_half<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, int, void*, size_t, hipStream_t);
+template int solver_half<float>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, int, void*, void*, void*, size_t, hipStream_t);
+template int solver_half<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, int, void*, void*, void*, size_t, hipStream_t);
 template int advect<float>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template int advect<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template int fbm_round<float>(const TsphParams&, TsphObject*, TsphControl*, void*, size_t, hipStream_t);

@@ -1,0 +1,141 @@
+// Neighbour iteration shared by the step and adjoint sweeps.
+//
+// Storage is in cell order.  The 27 (3D) / 9 (2D) cells of a particle's
+// neighbourhood are visited as 9 / 3 contiguous row runs: cells (x-R .. x+R)
+// of one (y, z) row have consecutive codes and therefore consecutive slots,
+// in the reference's template order (x fastest, sph_config.py:219-224) and
+// in-cell order.
+#pragma once
+
+#include "tsph_host.h"
+
+namespace tsph {
+
+// Visit every candidate slot of `cstart`'s object in the cells around c[].
+template <class T, class F>
+__device__ __forceinline__ void walk_cells(const KP<T>& k, const int* __restrict__ cstart,
+                                           const int* c, F&& f) {
+  const int R = k.R;
+  const int zr = k.dim == 3 ? R : 0;
+#pragma unroll 1
+  for (int dz = -zr; dz <= zr; ++dz) {
+#pragma unroll 1
+    for (int dy = -R; dy <= R; ++dy) {
+      int c0 = dim_encode(k, wadd(c[0], -R), wadd(c[1], dy), wadd(c[2], dz));
+      long lo = c0 < 1 ? 1 : (long)c0;
+      long hi = (long)c0 + 2 * R;
+      if (hi > k.node_num - 1) hi = k.node_num - 1;
+      if (lo > hi) continue;
+      int j0 = cstart[lo], j1 = cstart[hi + 1];
+      for (int j = j0; j < j1; ++j) f(j);
+    }
+  }
+}
+
+template <class T>
+__device__ __forceinline__ T rest_psi_of(const KP<T>& k, const T* rdens, int i) {
+  return k.solver == TSPH_VFSPH ? T(1) : rdens[i];
+}
+
+// Iterate a particle's fluid neighbours: the recorded list, or, when the list
+// overflowed, the cell walk with the same in-range test.
+template <class T, class L, class F>
+__device__ __forceinline__ void for_fluid_nbrs(const KP<T>& k, int i, int cap, int nf,
+                                               const L& list,
+                                               const int* __restrict__ cstart,
+                                               const V4<T>* __restrict__ pos, V4<T> p, F&& f) {
+  if (nf <= k.list_cap_f) {
+    for (int s = 0; s < nf; ++s) f(list[(long)s * cap + i]);
+  } else {
+    int c[3];
+    node_encode(k, p.x, p.y, p.z, c);
+    walk_cells(k, cstart, c, [&](int j) {
+      V4<T> q = pos[j];
+      T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
+      if (r2 > T(0) && r2 <= k.r2max) f(j);
+    });
+  }
+}
+template <class T, class F>
+__device__ __forceinline__ void for_bound_nbrs(const KP<T>& k, int i, int cap, int nbb,
+                                               const int* __restrict__ list,
+                                               const int* __restrict__ cstart,
+                                               const V4<T>* __restrict__ bpos, V4<T> p, F&& f) {
+  if (nbb <= k.list_cap_b) {
+    for (int s = 0; s < nbb; ++s) f(list[(long)s * cap + i]);
+  } else {
+    int c[3];
+    node_encode(k, p.x, p.y, p.z, c);
+    walk_cells(k, cstart, c, [&](int j) {
+      V4<T> q = bpos[j];
+      T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
+      if (r2 > T(0) && r2 <= k.r2max) f(j);
+    });
+  }
+}
+
+// Neighbour iteration of the list sweeps for one lane, in the recorded
+// (reference) order: fluid list, then boundary list.  The sweeps are bound
+// by the latency of dependent loads (list entry -> neighbour data), so the
+// lists are consumed in batches of kNb: all kNb indices are loaded, then all
+// their neighbour records, then the pair terms are applied in order.  A lane
+// whose list overflowed walks the cells with the same in-range test.
+constexpr int kNb = 8;
+
+template <class T, class Rec, class LoadF, class LoadB, class PairF, class PairB>
+__device__ __forceinline__ void sweep_lists(const KP<T>& k, int cap, int i, V4<T> p,
+                                            const int* __restrict__ fstart,
+                                            const int* __restrict__ bstart,
+                                            const V4<T>* __restrict__ fpos,
+                                            const V4<T>* __restrict__ bpos,
+                                            const int* __restrict__ nbr_f,
+                                            const int* __restrict__ nbr_b, int nf, int nbb,
+                                            LoadF&& load_f, LoadB&& load_b, PairF&& pair_f,
+                                            PairB&& pair_b) {
+  if (nf <= k.list_cap_f && nbb <= k.list_cap_b) {
+    for (int s0 = 0; s0 < nf; s0 += kNb) {
+      int j[kNb];
+      Rec d[kNb];
+#pragma unroll
+      for (int u = 0; u < kNb; ++u) j[u] = s0 + u < nf ? nbr_f[(long)(s0 + u) * cap + i] : -1;
+#pragma unroll
+      for (int u = 0; u < kNb; ++u)
+        if (j[u] >= 0) d[u] = load_f(j[u]);
+#pragma unroll
+      for (int u = 0; u < kNb; ++u)
+        if (j[u] >= 0) pair_f(d[u]);
+    }
+    for (int s0 = 0; s0 < nbb; s0 += kNb) {
+      int j[kNb];
+      Rec d[kNb];
+#pragma unroll
+      for (int u = 0; u < kNb; ++u) j[u] = s0 + u < nbb ? nbr_b[(long)(s0 + u) * cap + i] : -1;
+#pragma unroll
+      for (int u = 0; u < kNb; ++u)
+        if (j[u] >= 0) d[u] = load_b(j[u]);
+#pragma unroll
+      for (int u = 0; u < kNb; ++u)
+        if (j[u] >= 0) pair_b(d[u]);
+    }
+    return;
+  }
+  int c[3];
+  node_encode(k, p.x, p.y, p.z, c);
+  walk_cells(k, fstart, c, [&](int j) {
+    V4<T> q = fpos[j];
+    T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
+    if (r2 > T(0) && r2 <= k.r2max) pair_f(load_f(j));
+  });
+  walk_cells(k, bstart, c, [&](int j) {
+    V4<T> q = bpos[j];
+    T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
+    if (r2 > T(0) && r2 <= k.r2max) pair_b(load_b(j));
+  });
+}
+
+template <class T>
+struct Rec2 {
+  V4<T> a, b;
+};
+
+}  // namespace tsph

@@ -161,25 +161,55 @@ class Simulation:
         self.bound.adopt_swaps()
         self.bound_dirty = False
 
-    def _solve(self, solve):
-        P = ctypes.byref(self.params)
+    def forced_params(self, iters):
+        """A parameter block whose stop rule runs exactly `iters` Jacobi
+        iterations (iter_min = iter_max = iters, thresholds +inf)."""
+        p = _lib.TsphParams.from_buffer_copy(bytes(self.params))
+        p.iter_min = iters
+        p.iter_max = iters
+        p.divergence_threshold = float("inf")
+        p.compression_threshold = float("inf")
+        return p
+
+    def _solve(self, solve, tape=None, force=None):
+        """Run solve 0 (divergence) or 1 (density) to its stop rule.  With
+        `tape` (a list), each launched iteration k records vel_adv entering it
+        and its kappa into new tensors appended as (vel_adv, kappa); entries
+        past the iterations that ran are dropped.  `force` = n runs exactly n
+        iterations."""
+        params = self.params if force is None else self.forced_params(force)
+        P = ctypes.byref(params)
         ws, wsb = self._ws()
         st = self.stream_ptr()
         f, b = ctypes.byref(self.fluid.struct()), ctypes.byref(self.bound.struct())
         _lib.check(self.lib.tsph_solver_begin(P, self.ctrl_ptr(), solve, st), "tsph_solver_begin")
-        kmax = self.config.iter_threshold_max + 1
+        kmax = (self.config.iter_threshold_max if force is None else force) + 1
+        if force == 0:
+            kmax = 0
         k = 0
-        chunk = max(self.pred[solve], 1)
+        it = 0
+        chunk = max(self.pred[solve] if force is None else force, 1)
+        cap = self.fluid.t["pos"].shape[0]
         while k < kmax:
             for _ in range(min(chunk, kmax - k)):
-                self._call("solver_psi", self.lib.tsph_solver_psi, P, f, b, self.ctrl_ptr(), k, ws, wsb, st)
+                if tape is None:
+                    self._call("solver_psi", self.lib.tsph_solver_psi, P, f, b, self.ctrl_ptr(), k, ws, wsb, st)
+                else:
+                    tv = torch.empty(cap, 4, dtype=self.fluid.dtype, device=self.device)
+                    tk = torch.empty(cap, dtype=self.fluid.dtype, device=self.device)
+                    tape.append((tv, tk))
+                    self._call("solver_psi", self.lib.tsph_solver_psi_taped, P, f, b, self.ctrl_ptr(), k,
+                               ctypes.c_void_p(tv.data_ptr()), ctypes.c_void_p(tk.data_ptr()), ws, wsb, st)
                 self._call("solver_update", self.lib.tsph_solver_update, P, f, b, self.ctrl_ptr(), k, ws, wsb, st)
                 k += 1
             it, stop, _ = self._stop_state()
             if stop:
                 break
             chunk = 1
-        self.pred[solve] = max(it, 1)
+        if force is None:
+            self.pred[solve] = max(it, 1)
+        if tape is not None:
+            del tape[it:]
         return it
 
     def sph_step(self, stats=False):
