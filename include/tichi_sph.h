@@ -246,11 +246,13 @@ typedef struct TsphAdjoint {
   void *vbar_out;     /* V4: written by the sweeps that move the velocity state back */
   void *rhobar;       /* T: d loss / d sph_density (accumulated) */
   void *alphabar;     /* T: d loss / d alpha (accumulated) */
-  void *psibar;       /* T: d loss / d psi_adv before the clamp, per iteration */
-  void *nbar;         /* V4: d loss / d normal */
+  void *nbar;         /* V4: d loss / d normal (NULL: see tsph_adj_advect) */
   void *a1bar;        /* V4: d loss / d alpha_1 (xyz) and d alpha_2 (w) */
-  void *tape_vel_adv; /* V4: recorded vel_adv entering the iteration being reversed */
-  void *tape_kappa;   /* T: recorded kappa (psi_adv / alpha) of that iteration */
+  void *tape_vel_adv; /* V4: recorded vel_adv entering the iteration being reversed; its w
+                         lane is overwritten (the reverse consumes the record) */
+  void *kappa_next;   /* T: recorded kappa (psi_adv / alpha) of the iteration that the next
+                         tsph_adj_solver_iteration call reverses, or NULL if none: every
+                         call that writes the velocity adjoint stores it in vbar's w lane */
   void *tape_vel;     /* V4: recorded velocity after the divergence solve */
   void *tape_normal;  /* V4: recorded surface-tension normals */
   double *grad;       /* [2]: accumulated d loss / d dynamic_viscosity, d surface_tension_gamma */
@@ -278,15 +280,18 @@ int tsph_adj_begin(const TsphParams* p, TsphObject* fluid, TsphControl* ctrl, Ts
                    const int32_t* pid, const void* xbar_ref, const void* vbar_ref, void* stream);
 
 /* Reverse of one Jacobi iteration (IPPE_adv_psi + IPPE_psi_adv_non_negative
- * + IPPE_update_vel_adv, sph.py:196-244) whose record is adj->tape_vel_adv /
- * adj->tape_kappa: vbar -> vbar_out, accumulating xbar, rhobar, alphabar. */
+ * + IPPE_update_vel_adv, sph.py:196-244) whose record is adj->tape_vel_adv,
+ * with that iteration's kappa in vbar's w lane: vbar -> vbar_out,
+ * accumulating xbar, rhobar, alphabar. */
 int tsph_adj_solver_iteration(const TsphParams* p, TsphObject* fluid, TsphObject* bound,
                               TsphControl* ctrl, TsphAdjoint* adj, void* workspace,
                               size_t ws_bytes, void* stream);
 
 /* Reverse of the advection (viscosity, normals, surface tension and
  * vel_adv = vel + dt * acce_adv; sph.py:108-166, 247-250): vbar -> vbar_out,
- * accumulating xbar, rhobar and adj->grad. */
+ * accumulating xbar, rhobar and adj->grad.  adj->nbar may be NULL when
+ * surface_tension_gamma == 0 and d loss / d gamma is not wanted: every
+ * surface-tension term then vanishes and that reverse sweep is skipped. */
 int tsph_adj_advect(const TsphParams* p, TsphObject* fluid, TsphControl* ctrl, TsphAdjoint* adj,
                     void* workspace, size_t ws_bytes, void* stream);
 

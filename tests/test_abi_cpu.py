@@ -37,7 +37,7 @@ def test_struct_layout_matches_header(tmp_path):
         "TsphObject": ["n", "pos", "pressure", "pid", "cell_structured", "cell_start", "nbr_nb", "pid_b"],
         "TsphControl": ["dt", "it", "stop", "general_flag", "ke", "run_u", "err_div", "err_incom",
                         "overflow_f", "counter2"],
-        "TsphAdjoint": ["xbar", "vbar_out", "psibar", "a1bar", "tape_kappa", "tape_normal", "grad"],
+        "TsphAdjoint": ["xbar", "vbar_out", "nbar", "a1bar", "kappa_next", "tape_normal", "grad"],
     }
     src = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HDR}"', "int main(void){"]
     for s, fl in fields.items():

@@ -7,5 +7,5 @@ Fluid, Ngrid, NeighborSearch, the sph.py kernel functions and sph_step().
 """
 from .config import Config, read_param  # noqa: F401
 from .sph_obj import Fluid, Ngrid  # noqa: F401
-from .solver import Simulation, StepStats, load_simulation, simulation_from_dicts  # noqa: F401
+from .solver import EventTimer, Simulation, StepStats, load_simulation, simulation_from_dicts  # noqa: F401
 from .NeighborSearch import NeighborSearch  # noqa: F401

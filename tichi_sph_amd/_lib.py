@@ -70,8 +70,8 @@ class TsphControl(ctypes.Structure):
 
 
 # Reference kernel ids (sph.py), include/tichi_sph.h
-_ADJ_PTRS = ["xbar", "vbar", "vbar_out", "rhobar", "alphabar", "psibar", "nbar", "a1bar",
-             "tape_vel_adv", "tape_kappa", "tape_vel", "tape_normal", "grad"]
+_ADJ_PTRS = ["xbar", "vbar", "vbar_out", "rhobar", "alphabar", "nbar", "a1bar",
+             "tape_vel_adv", "kappa_next", "tape_vel", "tape_normal", "grad"]
 
 
 class TsphAdjoint(ctypes.Structure):

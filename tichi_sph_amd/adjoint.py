@@ -65,13 +65,15 @@ class DiffRunner:
         dt_, dev = f.dtype, sim.device
         self.buf = {name: torch.zeros(cap, 4, dtype=dt_, device=dev)
                     for name in ("xbar", "vbar", "vbar_out", "nbar", "a1bar")}
-        for name in ("rhobar", "alphabar", "psibar"):
+        for name in ("rhobar", "alphabar"):
             self.buf[name] = torch.zeros(cap, dtype=dt_, device=dev)
         self.grad = torch.zeros(2, dtype=torch.float64, device=dev)
         self.adj = _lib.TsphAdjoint()
         for name, t in self.buf.items():
             setattr(self.adj, name, t.data_ptr())
         self.adj.grad = self.grad.data_ptr()
+        # False skips the surface-tension reverse; allowed only while gamma == 0
+        self.surface_tension_grad = True
         self.force = None     # optional list of (div_iter, incom_iter) per step to replay
         self.step_index = 0
         self.iter_log = []    # (div_iter, incom_iter) of every forward step since the last reset
@@ -162,7 +164,8 @@ class DiffRunner:
     def reverse(self, tp, xbar_ref, vbar_ref):
         """Reverse of one recorded step.  xbar_ref / vbar_ref ([n, 4], pid
         order) hold d loss / d (pos, vel) after the step on entry and before
-        it on return.  adj.grad accumulates d loss / d (mu, gamma)."""
+        it on return.  adj.grad accumulates d loss / d (mu, gamma).  The tape
+        is consumed (its vel_adv records carry psi_bar afterwards)."""
         sim = self.sim
         f, b = sim.fluid, sim.bound
         n = f.part_num
@@ -176,21 +179,30 @@ class DiffRunner:
         fs, bs = ctypes.byref(f.struct()), ctypes.byref(b.struct())
         A = ctypes.byref(self.adj)
         vp = ctypes.c_void_p
+        def last_kappa(pairs):
+            return pairs[-1][1].data_ptr() if pairs else None
+
         sim._call("relist", self.lib.tsph_relist, P, fs, bs, sim.ctrl_ptr(), ws, wsb, stp)
+        self.adj.kappa_next = last_kappa(tp.iters[1])
         sim._call("adj_begin", self.lib.tsph_adj_begin, P, fs, sim.ctrl_ptr(), A, vp(tp.pid.data_ptr()),
                   vp(xbar_ref.data_ptr()), vp(vbar_ref.data_ptr()), stp)
 
         def iterations(pairs):
-            for tv, tk in reversed(pairs):
-                self.adj.tape_vel_adv = tv.data_ptr()
-                self.adj.tape_kappa = tk.data_ptr()
+            # each call reads its iteration's kappa from vbar.w and stores the
+            # next (earlier) iteration's kappa into vbar_out.w
+            for k in range(len(pairs) - 1, -1, -1):
+                self.adj.tape_vel_adv = pairs[k][0].data_ptr()
+                self.adj.kappa_next = pairs[k - 1][1].data_ptr() if k > 0 else None
                 sim._call("adj_solver_iteration", self.lib.tsph_adj_solver_iteration, P, fs, bs,
                           sim.ctrl_ptr(), A, ws, wsb, stp)
                 self._swap_v()
 
         iterations(tp.iters[1])
+        self.adj.kappa_next = last_kappa(tp.iters[0])
         self.adj.tape_vel = tp.vel.data_ptr()
         self.adj.tape_normal = tp.normal.data_ptr()
+        skip_st = not self.surface_tension_grad and sim.params.surface_tension_gamma == 0.0
+        self.adj.nbar = None if skip_st else self.buf["nbar"].data_ptr()
         sim._call("adj_advect", self.lib.tsph_adj_advect, P, fs, sim.ctrl_ptr(), A, ws, wsb, stp)
         self._swap_v()
         iterations(tp.iters[0])
@@ -208,6 +220,7 @@ class _Rollout(torch.autograd.Function):
     @staticmethod
     def forward(ctx, v0, mu, gamma, runner, steps, every):
         runner.set_params(mu.item(), gamma.item())
+        runner.surface_tension_grad = bool(gamma.requires_grad) or gamma.item() != 0.0
         runner.set_velocity_ref(v0.detach())
         runner.step_index = 0
         runner.iter_log = []

@@ -94,15 +94,17 @@ __device__ __forceinline__ void add_J(T& ax, T& ay, T& az, const D2<T>& d, T dx,
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_adj_begin(int n, const int* __restrict__ pid, const V4<T>* __restrict__ xref,
-                const V4<T>* __restrict__ vref, V4<T>* __restrict__ xbar, V4<T>* __restrict__ vbar,
-                T* __restrict__ rhobar, T* __restrict__ alphabar, const TsphControl* ctrl) {
+                const V4<T>* __restrict__ vref, const T* __restrict__ kappa_next,
+                V4<T>* __restrict__ xbar, V4<T>* __restrict__ vbar, T* __restrict__ rhobar,
+                T* __restrict__ alphabar, const TsphControl* ctrl) {
   int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i >= n) return;
   const T dt = (T)ctrl->dt;
   int p = pid[i];
   V4<T> xb = xref[p], vb = vref[p];
   xbar[i] = v4(xb.x, xb.y, xb.z);
-  vbar[i] = v4(vb.x + dt * xb.x, vb.y + dt * xb.y, vb.z + dt * xb.z);
+  vbar[i] = v4(vb.x + dt * xb.x, vb.y + dt * xb.y, vb.z + dt * xb.z,
+               kappa_next ? kappa_next[i] : T(0));
   rhobar[i] = T(0);
   alphabar[i] = T(0);
 }
@@ -126,52 +128,52 @@ __global__ void __launch_bounds__(kBlock)
 //   kappa_bar_l = -(1/dt) sum_j X_j G(x_lj) . (w_l - w_j)
 //   x_bar_l    += -(1/dt) sum_j J (kappa_l X_j + kappa_j X_l) (w_l - w_j)
 // (boundary neighbours: w_j = 0, kappa_j = 0).  Then kappa = psi / alpha and
-// the clamp psi = max(psi_pre, 0) (sph.py:219-226) give psi_bar (written),
-// alpha_bar and rho_bar (psi_pre = rho - rho_0 + ..., sph.py:196-199).
-// ---------------------------------------------------------------------------
+// the clamp psi = max(psi_pre, 0) (sph.py:219-226) give psi_bar, alpha_bar
+// and rho_bar (psi_pre = rho - rho_0 + ..., sph.py:196-199).
+// DFSPH: m = X = pos.w.  The iteration's kappa rides in vbar.w (packed by
+// the kernel that wrote vbar), so a neighbour costs two 16-byte gathers;
+// psi_bar is written into the w lane of the iteration's vel_adv record,
+// where the second half gathers it.
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_adj_kappa(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
-                const T* __restrict__ fmass, const T* __restrict__ alpha,
-                const T* __restrict__ tkappa, const V4<T>* __restrict__ vbar,
+                const T* __restrict__ alpha, const V4<T>* __restrict__ vbar,
                 const int* __restrict__ fstart, const V4<T>* __restrict__ bpos,
                 const int* __restrict__ bstart, const int* __restrict__ nbr_f,
                 const int* __restrict__ nbr_b, const int* __restrict__ nbr_nf,
-                const int* __restrict__ nbr_nb, T* __restrict__ psibar, T* __restrict__ rhobar,
+                const int* __restrict__ nbr_nb, V4<T>* __restrict__ tvel, T* __restrict__ rhobar,
                 T* __restrict__ alphabar, V4<T>* __restrict__ xbar, const TsphControl* ctrl) {
   const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i >= n) return;
   const T dt = (T)ctrl->dt;
   const V4<T> p = fpos[i];
   const T Xi = p.w;
-  const T mi = fmass[i];
   const V4<T> u = vbar[i];
-  const T wx = u.x / mi, wy = u.y / mi, wz = u.z / mi;
-  const T ki = tkappa[i];
+  const T rmi = T(1) / Xi;
+  const T wx = u.x * rmi, wy = u.y * rmi, wz = u.z * rmi;
+  const T ki = u.w;
   T kb = 0, ax = 0, ay = 0, az = 0;
-  auto term = [&](const Rec2<T>& d) {
-    const V4<T>& q = d.a;  // q.w = X_j
-    const V4<T>& wj = d.b; // xyz = u_j / m_j, w = kappa_j
+  auto term = [&](const Rec2<T>& d, bool fluid) {
+    const V4<T>& q = d.a;   // q.w = X_j = m_j
+    const V4<T>& uj = d.b;  // xyz = u_j, w = kappa_j (zero for the boundary)
     T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     D2<T> e = kern_d2(k, dx * dx + dy * dy + dz * dz);
-    T mx = wx - wj.x, my = wy - wj.y, mz = wz - wj.z;
+    T rmj = fluid ? T(1) / q.w : T(0);
+    T mx = wx - uj.x * rmj, my = wy - uj.y * rmj, mz = wz - uj.z * rmj;
     kb += q.w * e.g * (dx * mx + dy * my + dz * mz);
-    T c = ki * q.w + wj.w * Xi;
+    T c = ki * q.w + uj.w * Xi;
     add_J(ax, ay, az, e, dx, dy, dz, c * mx, c * my, c * mz);
   };
   sweep_lists<T, Rec2<T>>(
       k, cap, i, p, fstart, bstart, fpos, bpos, nbr_f, nbr_b, nbr_nf[i], nbr_nb[i],
-      [&](int j) {
-        V4<T> uj = vbar[j];
-        T mj = fmass[j];
-        return Rec2<T>{fpos[j], v4(uj.x / mj, uj.y / mj, uj.z / mj, tkappa[j])};
-      },
-      [&](int j) { return Rec2<T>{bpos[j], v4(T(0), T(0), T(0), T(0))}; }, term, term);
+      [&](int j) { return Rec2<T>{fpos[j], vbar[j]}; },
+      [&](int j) { return Rec2<T>{bpos[j], v4(T(0), T(0), T(0), T(0))}; },
+      [&](const Rec2<T>& d) { term(d, true); }, [&](const Rec2<T>& d) { term(d, false); });
   const T s = -(T(1) / dt);
   kb *= s;
   const T al = alpha[i];
   const T pb = ki > T(0) ? kb / al : T(0);
-  psibar[i] = pb;
+  tvel[i].w = pb;
   rhobar[i] += pb;
   alphabar[i] += -kb * ki / al;
   V4<T> xb = xbar[i];
@@ -188,8 +190,8 @@ __global__ void __launch_bounds__(kBlock)
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_adj_vel(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
-              const V4<T>* __restrict__ tvel, const T* __restrict__ psibar,
-              const V4<T>* __restrict__ vbar, const int* __restrict__ fstart,
+              const V4<T>* __restrict__ tvel, const V4<T>* __restrict__ vbar,
+              const T* __restrict__ kappa_next, const int* __restrict__ fstart,
               const V4<T>* __restrict__ bpos, const int* __restrict__ bstart,
               const int* __restrict__ nbr_f, const int* __restrict__ nbr_b,
               const int* __restrict__ nbr_nf, const int* __restrict__ nbr_nb,
@@ -199,8 +201,8 @@ __global__ void __launch_bounds__(kBlock)
   const T dt = (T)ctrl->dt;
   const V4<T> p = fpos[i];
   const T Xi = p.w;
-  const V4<T> vi = tvel[i];
-  const T pbi = psibar[i];
+  const V4<T> vi = tvel[i];  // w = psi_bar of this particle
+  const T pbi = vi.w;
   T vx = 0, vy = 0, vz = 0, ax = 0, ay = 0, az = 0;
   auto term = [&](const Rec2<T>& d) {
     const V4<T>& q = d.a;   // q.w = X_j
@@ -216,14 +218,11 @@ __global__ void __launch_bounds__(kBlock)
   };
   sweep_lists<T, Rec2<T>>(
       k, cap, i, p, fstart, bstart, fpos, bpos, nbr_f, nbr_b, nbr_nf[i], nbr_nb[i],
-      [&](int j) {
-        V4<T> v = tvel[j];
-        v.w = psibar[j];
-        return Rec2<T>{fpos[j], v};
-      },
+      [&](int j) { return Rec2<T>{fpos[j], tvel[j]}; },
       [&](int j) { return Rec2<T>{bpos[j], v4(T(0), T(0), T(0), T(0))}; }, term, term);
   const V4<T> u = vbar[i];
-  vbar_out[i] = v4(u.x + dt * vx, u.y + dt * vy, u.z + dt * vz);
+  vbar_out[i] = v4(u.x + dt * vx, u.y + dt * vy, u.z + dt * vz,
+                   kappa_next ? kappa_next[i] : T(0));
   V4<T> xb = xbar[i];
   xbar[i] = v4(xb.x + dt * ax, xb.y + dt * ay, xb.z + dt * az);
 }
@@ -335,9 +334,10 @@ __global__ void __launch_bounds__(kBlock)
                     const int* __restrict__ bstart, const V4<T>* __restrict__ bpos,
                     const int* __restrict__ nbr_f, const int* __restrict__ nbr_nf,
                     const V4<T>* __restrict__ tvel, const V4<T>* __restrict__ nbar,
-                    const V4<T>* __restrict__ vbar, V4<T>* __restrict__ vbar_out,
-                    T* __restrict__ rhobar, V4<T>* __restrict__ xbar,
-                    double* __restrict__ partial, const TsphControl* ctrl) {
+                    const V4<T>* __restrict__ vbar, const T* __restrict__ kappa_next,
+                    V4<T>* __restrict__ vbar_out, T* __restrict__ rhobar,
+                    V4<T>* __restrict__ xbar, double* __restrict__ partial,
+                    const TsphControl* ctrl) {
   __shared__ double lds[kWaves];
   const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   double msum = 0;
@@ -346,7 +346,7 @@ __global__ void __launch_bounds__(kBlock)
     const V4<T> p = fpos[i];
     const V4<T> vi = tvel[i];
     const V4<T> ui = vbar[i];
-    const V4<T> nbi = nbar[i];
+    const V4<T> nbi = nbar ? nbar[i] : V4<T>{};
     const T ax_ = dt * ui.x, ay_ = dt * ui.y, az_ = dt * ui.z;
     const T Vi = vol[i];
     const T fmu_i = k.two_d2 / rdens[i];  // fv_i / mu
@@ -395,10 +395,10 @@ __global__ void __launch_bounds__(kBlock)
           v.w = vol[j];
           V4<T> u = vbar[j];
           V4<T> a = v4(dt * u.x, dt * u.y, dt * u.z, k.two_d2 * k.mu / rdens[j]);
-          return Rec4<T>{fpos[j], v, a, nbar[j]};
+          return Rec4<T>{fpos[j], v, a, nbar ? nbar[j] : V4<T>{}};
         },
         [&](int j) { return Rec4<T>{}; }, term, [&](const Rec4<T>&) {});
-    vbar_out[i] = v4(ui.x + vbx, ui.y + vby, ui.z + vbz);
+    vbar_out[i] = v4(ui.x + vbx, ui.y + vby, ui.z + vbz, kappa_next ? kappa_next[i] : T(0));
     rhobar[i] += -Vb * Vi / dens[i];
     V4<T> xb = xbar[i];
     xbar[i] = v4(xb.x + xbx, xb.y + xby, xb.z + xbz);
@@ -461,8 +461,9 @@ __global__ void __launch_bounds__(kBlock)
   sweep_lists<T, Rec4<T>>(
       k, cap, i, p, fstart, bstart, fpos, bpos, nbr_f, nbr_b, nbr_nf[i], nbr_nb[i],
       [&](int j) {
-        V4<T> extra = v4(rhobar[j], fmass[j], T(1), T(0));
-        return Rec4<T>{fpos[j], a1bar[j], extra, V4<T>{}};
+        V4<T> q = fpos[j];
+        V4<T> extra = v4(rhobar[j], q.w, T(1), T(0));  // DFSPH: m_j = X_j
+        return Rec4<T>{q, a1bar[j], extra, V4<T>{}};
       },
       [&](int j) {
         V4<T> extra = v4(T(0), bmass[j], T(0), T(0));
@@ -508,8 +509,8 @@ int adj_begin(const TsphParams& prm, TsphObject* f, TsphControl* ctrl, TsphAdjoi
   (void)prm;
   if (f->n == 0) return 0;
   hipLaunchKernelGGL(k_adj_begin<T>, dim3(blocks_for(f->n)), dim3(kBlock), 0, st, f->n, pid,
-                     (const V4<T>*)xref, (const V4<T>*)vref, (V4<T>*)a->xbar, (V4<T>*)a->vbar,
-                     (T*)a->rhobar, (T*)a->alphabar, ctrl);
+                     (const V4<T>*)xref, (const V4<T>*)vref, (const T*)a->kappa_next,
+                     (V4<T>*)a->xbar, (V4<T>*)a->vbar, (T*)a->rhobar, (T*)a->alphabar, ctrl);
   TSPH_LAUNCH_CHECK("adj_begin");
   return 0;
 }
@@ -534,14 +535,13 @@ int adj_solver_iteration(const TsphParams& prm, TsphObject* f, TsphObject* b, Ts
   if (f->n == 0) return 0;
   int nb = blocks_for(f->n);
   hipLaunchKernelGGL(k_adj_kappa<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
-                     (const V4<T>*)f->pos, (const T*)f->mass, (const T*)f->alpha,
-                     (const T*)a->tape_kappa, (const V4<T>*)a->vbar, f->cell_start,
-                     (const V4<T>*)b->pos, b->cell_start, f->nbr_f, f->nbr_b, f->nbr_nf,
-                     f->nbr_nb, (T*)a->psibar, (T*)a->rhobar, (T*)a->alphabar, (V4<T>*)a->xbar,
-                     ctrl);
+                     (const V4<T>*)f->pos, (const T*)f->alpha, (const V4<T>*)a->vbar,
+                     f->cell_start, (const V4<T>*)b->pos, b->cell_start, f->nbr_f, f->nbr_b,
+                     f->nbr_nf, f->nbr_nb, (V4<T>*)a->tape_vel_adv, (T*)a->rhobar,
+                     (T*)a->alphabar, (V4<T>*)a->xbar, ctrl);
   hipLaunchKernelGGL(k_adj_vel<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
-                     (const V4<T>*)f->pos, (const V4<T>*)a->tape_vel_adv, (const T*)a->psibar,
-                     (const V4<T>*)a->vbar, f->cell_start, (const V4<T>*)b->pos, b->cell_start,
+                     (const V4<T>*)f->pos, (const V4<T>*)a->tape_vel_adv, (const V4<T>*)a->vbar,
+                     (const T*)a->kappa_next, f->cell_start, (const V4<T>*)b->pos, b->cell_start,
                      f->nbr_f, f->nbr_b, f->nbr_nf, f->nbr_nb, (V4<T>*)a->vbar_out,
                      (V4<T>*)a->xbar, ctrl);
   TSPH_LAUNCH_CHECK("adj_solver_iteration");
@@ -557,18 +557,22 @@ int adj_advect(const TsphParams& prm, TsphObject* f, TsphControl* ctrl, TsphAdjo
                "tsph_adj_advect: workspace too small");
   if (f->n == 0) return 0;
   int nb = blocks_for(f->n);
-  hipLaunchKernelGGL(k_adj_surface_tension<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n,
-                     (const V4<T>*)f->pos, (const T*)f->mass, (const T*)f->sph_density,
-                     (const T*)f->rest_density, (const T*)f->volume_frac, f->cell_start,
-                     (const V4<T>*)a->tape_normal, (const V4<T>*)a->vbar, (V4<T>*)a->nbar,
-                     (T*)a->rhobar, (V4<T>*)a->xbar, L.partial, ctrl);
-  hipLaunchKernelGGL(k_adj_grad_add, dim3(1), dim3(kBlock), 0, st, L.partial, nb, a->grad, 1);
+  // nbar == NULL: gamma == 0 and its gradient is not wanted, so every term
+  // of the surface-tension reverse vanishes and it is skipped
+  if (a->nbar) {
+    hipLaunchKernelGGL(k_adj_surface_tension<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n,
+                       (const V4<T>*)f->pos, (const T*)f->mass, (const T*)f->sph_density,
+                       (const T*)f->rest_density, (const T*)f->volume_frac, f->cell_start,
+                       (const V4<T>*)a->tape_normal, (const V4<T>*)a->vbar, (V4<T>*)a->nbar,
+                       (T*)a->rhobar, (V4<T>*)a->xbar, L.partial, ctrl);
+    hipLaunchKernelGGL(k_adj_grad_add, dim3(1), dim3(kBlock), 0, st, L.partial, nb, a->grad, 1);
+  }
   hipLaunchKernelGGL(k_adj_viscosity<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
                      (const V4<T>*)f->pos, (const T*)f->vol, (const T*)f->sph_density,
                      (const T*)f->rest_density, f->cell_start, (const int*)nullptr,
                      (const V4<T>*)nullptr, f->nbr_f, f->nbr_nf, (const V4<T>*)a->tape_vel,
-                     (const V4<T>*)a->nbar, (const V4<T>*)a->vbar, (V4<T>*)a->vbar_out,
-                     (T*)a->rhobar, (V4<T>*)a->xbar, L.partial, ctrl);
+                     (const V4<T>*)a->nbar, (const V4<T>*)a->vbar, (const T*)a->kappa_next,
+                     (V4<T>*)a->vbar_out, (T*)a->rhobar, (V4<T>*)a->xbar, L.partial, ctrl);
   hipLaunchKernelGGL(k_adj_grad_add, dim3(1), dim3(kBlock), 0, st, L.partial, nb, a->grad, 0);
   TSPH_LAUNCH_CHECK("adj_advect");
   return 0;

@@ -276,8 +276,7 @@ int tsph_adj_solver_iteration(const TsphParams* p, TsphObject* f, TsphObject* b,
   if (int rc = check_obj(f, "tsph_adj_solver_iteration")) return rc;
   if (int rc = check_obj(b, "tsph_adj_solver_iteration")) return rc;
   if (int rc = check_adj(p, adj, "tsph_adj_solver_iteration")) return rc;
-  TSPH_REQUIRE(adj->tape_vel_adv && adj->tape_kappa, TSPH_EINVAL,
-               "tsph_adj_solver_iteration: null tape");
+  TSPH_REQUIRE(adj->tape_vel_adv, TSPH_EINVAL, "tsph_adj_solver_iteration: null tape");
   return DISPATCH(p, adj_solver_iteration, *p, f, b, ctrl, adj, ws, wsb, (hipStream_t)stream);
 }
 
@@ -286,8 +285,10 @@ int tsph_adj_advect(const TsphParams* p, TsphObject* f, TsphControl* ctrl, TsphA
   if (int rc = check_params(p)) return rc;
   if (int rc = check_obj(f, "tsph_adj_advect")) return rc;
   if (int rc = check_adj(p, adj, "tsph_adj_advect")) return rc;
-  TSPH_REQUIRE(adj->tape_vel && adj->tape_normal && adj->grad, TSPH_EINVAL,
+  TSPH_REQUIRE(adj->tape_vel && adj->grad && (adj->tape_normal || !adj->nbar), TSPH_EINVAL,
                "tsph_adj_advect: null tape or grad");
+  TSPH_REQUIRE(adj->nbar || p->surface_tension_gamma == 0.0, TSPH_EINVAL,
+               "tsph_adj_advect: nbar may be NULL only when surface_tension_gamma == 0");
   return DISPATCH(p, adj_advect, *p, f, ctrl, adj, ws, wsb, (hipStream_t)stream);
 }
 

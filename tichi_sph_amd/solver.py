@@ -41,6 +41,38 @@ class StepStats:
                 f"compression={self.compression:.4g})")
 
 
+class EventTimer:
+    """Per-call HIP-event timing: each timed library call records a start and
+    an end event on the launch stream.  Events come from a pool created up
+    front (`reserve`), so timing adds no event creation inside a timed loop."""
+
+    def __init__(self, reserve=0):
+        self.pool = []
+        self.used = 0
+        self.records = []  # (name, start event, end event)
+        self.reserve(reserve)
+
+    def reserve(self, n):
+        while len(self.pool) < n:
+            self.pool.append(torch.cuda.Event(enable_timing=True))
+
+    def pair(self):
+        if self.used + 2 > len(self.pool):
+            self.reserve(self.used + 256)
+        e0, e1 = self.pool[self.used], self.pool[self.used + 1]
+        self.used += 2
+        return e0, e1
+
+    def per_name(self):
+        """{name: [total ms, calls]} (synchronises on the recorded events)."""
+        out = {}
+        for name, e0, e1 in self.records:
+            d = out.setdefault(name, [0.0, 0])
+            d[0] += e0.elapsed_time(e1)
+            d[1] += 1
+        return out
+
+
 class Simulation:
     """Owns the two particle objects (fluid uid 0, boundary uid 1), the
     control block, the workspace and the reference-layout Ngrid."""
@@ -71,7 +103,7 @@ class Simulation:
             raise Exception(f"solver_iter_threshold_max must be < {_lib.TSPH_MAX_ITER_LOG}")
         self.set_dt(config.dt)
         self.time_count = 0.0
-        # optional per-call HIP-event timing: a list receiving (name, start, end)
+        # optional per-call HIP-event timing (an EventTimer)
         self.timer = None
 
     # ----------------------------------------------------------------- helpers
@@ -82,12 +114,11 @@ class Simulation:
         if self.timer is None:
             _lib.check(fn(*args), name)
             return
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
+        e0, e1 = self.timer.pair()
         e0.record()
         _lib.check(fn(*args), name)
         e1.record()
-        self.timer.append((name, e0, e1))
+        self.timer.records.append((name, e0, e1))
 
     def ctrl_ptr(self):
         return ctypes.c_void_p(self.ctrl.data_ptr())
