@@ -180,6 +180,18 @@ def test_fused_step_matches_oracle(name):
               f"oracle32={rel_err(o.get(0, fld), o64.get(0, fld)):.2e}", flush=True)
 
 
+def test_fused_step_with_overflowing_lists_matches_oracle():
+    """List capacities of 8 / 4 overflow for every bulk particle: pass A then
+    takes its combined walk and the sweeps their cell walks; results must
+    not change."""
+    sim, o = make("dambreak_3d_8k", list_cap_f=8, list_cap_b=4)
+    s = sim.sph_step(stats=True)
+    st = o.step(O.STEP_CFL)
+    assert s.overflow_f > 0 and s.overflow_b > 0
+    assert (s.div_iter, s.incom_iter) == (st.div_iter, st.incom_iter)
+    compare_fields(sim, o, FIELDS_F + ["alpha_1", "alpha_2", "normal"], label="overflow")
+
+
 def test_fused_multi_step_statistics():
     sim, o = make("dambreak_2d_8k")
     for step in range(20):

@@ -10,11 +10,12 @@ Jacobi iteration counts, the psi clamps (sph.py:219-226), the alpha clamp
 between such events the step is smooth (the cubic spline is C^1 with
 W'(h) = W''(h) = 0) and the derivative is exact.
 
-Memory.  The forward pass keeps only checkpoints: the persistent state
-(pos, vel, mass, rest density, rest volume, volume fractions, pid and the
-control block) every `every` steps.  The backward pass replays each segment
-from its checkpoint while recording a tape per step, then reverses the
-segment step by step.  A tape holds the sorted positions, masses, rest
+Memory.  The forward pass keeps checkpoints: the persistent state (pos,
+vel, mass, rest density, rest volume, volume fractions, pid and the control
+block) every `every` steps, and the tapes of the last segment, which the
+backward pass reverses first.  The backward pass replays every other
+segment from its checkpoint while recording a tape per step, then reverses
+the segment step by step.  A tape holds the sorted positions, masses, rest
 densities, pids and cell table, vel_adv and kappa of every Jacobi iteration,
 and the advection's velocity and normals.  Neighbour lists, densities and
 alpha factors are not stored: `tsph_relist` recomputes them bit-identically
@@ -35,7 +36,8 @@ from . import _lib
 class StepTape:
     """Everything the reverse of one step reads that is not recomputed."""
 
-    __slots__ = ("pos", "mass", "rest_density", "pid", "cell_start", "vel", "normal", "iters")
+    __slots__ = ("pos", "mass", "rest_density", "pid", "cell_start", "vel", "normal", "iters",
+                 "calls")
 
     def __init__(self):
         self.iters = ([], [])  # per solve: [(vel_adv entering it, kappa), ...]
@@ -156,6 +158,8 @@ class DiffRunner:
             tp.normal = f.t["normal"][:n].clone()
         i_it = sim._solve(1, tape=tp.iters[1] if tape else None, force=force[1])
         sim._call("tail", sim.lib.tsph_tail, P, fs, sim.ctrl_ptr(), 0, ws, wsb, stp)
+        if tape:
+            tp.calls = sim.calls  # no later call: lists and density / alpha still resident
         self.iter_log.append((d_it, i_it))
         self.step_index += 1
         return tp
@@ -182,7 +186,8 @@ class DiffRunner:
         def last_kappa(pairs):
             return pairs[-1][1].data_ptr() if pairs else None
 
-        sim._call("relist", self.lib.tsph_relist, P, fs, bs, sim.ctrl_ptr(), ws, wsb, stp)
+        if tp.calls != sim.calls:
+            sim._call("relist", self.lib.tsph_relist, P, fs, bs, sim.ctrl_ptr(), ws, wsb, stp)
         self.adj.kappa_next = last_kappa(tp.iters[1])
         sim._call("adj_begin", self.lib.tsph_adj_begin, P, fs, sim.ctrl_ptr(), A, vp(tp.pid.data_ptr()),
                   vp(xbar_ref.data_ptr()), vp(vbar_ref.data_ptr()), stp)
@@ -225,11 +230,18 @@ class _Rollout(torch.autograd.Function):
         runner.step_index = 0
         runner.iter_log = []
         ckpts = []
+        last = ((steps - 1) // every) * every  # first step of the last segment
+        last_tapes = []
         for t in range(steps):
             if t % every == 0:
                 ckpts.append(runner.checkpoint())
-            runner.step()
+            # the last segment is reversed first: record its tapes now instead
+            # of replaying it in the backward pass
+            tp = runner.step(tape=t >= last)
+            if tp is not None:
+                last_tapes.append(tp)
         ctx.runner, ctx.ckpts, ctx.steps, ctx.every = runner, ckpts, steps, every
+        ctx.last_tapes = last_tapes
         ctx.mu, ctx.gamma = float(mu.item()), float(gamma.item())
         ctx.v0_dtype = v0.dtype
         return runner.positions_ref(), runner.velocities_ref()
@@ -255,8 +267,11 @@ class _Rollout(torch.autograd.Function):
         for s in reversed(range(len(ctx.ckpts))):
             t0 = s * ctx.every
             t1 = min(t0 + ctx.every, ctx.steps)
-            runner.restore(ctx.ckpts[s])
-            tapes = [runner.step(tape=True) for _ in range(t0, t1)]
+            if s == len(ctx.ckpts) - 1:
+                tapes, ctx.last_tapes = ctx.last_tapes, None
+            else:
+                runner.restore(ctx.ckpts[s])
+                tapes = [runner.step(tape=True) for _ in range(t0, t1)]
             for tp in reversed(tapes):
                 runner.reverse(tp, xbar, vbar)
             del tapes

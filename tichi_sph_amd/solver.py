@@ -105,12 +105,14 @@ class Simulation:
         self.time_count = 0.0
         # optional per-call HIP-event timing (an EventTimer)
         self.timer = None
+        self.calls = 0  # library calls issued through _call (a change counter)
 
     # ----------------------------------------------------------------- helpers
     def stream_ptr(self):
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
     def _call(self, name, fn, *args):
+        self.calls += 1
         if self.timer is None:
             _lib.check(fn(*args), name)
             return
