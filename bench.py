@@ -139,6 +139,62 @@ def adjoint_leg(scen, steps, every):
             "kernel_ms_per_step": {k: round(x[0] / steps, 4) for k, x in sorted(per.items())}}
 
 
+DIFF_WORKLOAD = "diff_3d_200k"
+
+
+def differentiable_leg(scen, steps=200, every=20):
+    """The differentiable dam break: 200k fluid particles, `steps` steps
+    checkpointed every `every`; loss = mean |x - x_target|^2 with the target
+    cloud from a forward run at mu = 0.02, gamma = 0.05; gradients w.r.t.
+    mu, gamma and the initial velocity field, from mu = 0.01, gamma = 0.07
+    and v0 ~ N(0, 0.01^2) per component (numpy default_rng(0))."""
+    import numpy as np
+    import torch
+    from tichi_sph_amd import load_simulation
+    from tichi_sph_amd.adjoint import DiffRunner, rollout
+
+    sim = load_simulation(os.path.join(scen, DIFF_WORKLOAD + ".config.json"),
+                          os.path.join(scen, DIFF_WORKLOAD + ".json"), cfl=False)
+    r = DiffRunner(sim)
+    st0 = r.checkpoint()
+    nf, d = sim.fluid.part_num, sim.config.dim
+    dev, dt_ = sim.device, sim.fluid.dtype
+    v0 = torch.as_tensor(np.random.default_rng(0).normal(0.0, 0.01, (nf, d)), dtype=dt_, device=dev)
+
+    def forward(mu, gam):
+        r.restore(st0)
+        r.set_params(mu, gam)
+        r.set_velocity_ref(v0)
+        for _ in range(steps):
+            r.step()
+        return r.positions_ref()
+
+    target = forward(0.02, 0.05).clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    forward(0.01, 0.07)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    r.restore(st0)
+    v = v0.clone().requires_grad_(True)
+    mu = torch.tensor(0.01, dtype=dt_, device=dev, requires_grad=True)
+    gam = torch.tensor(0.07, dtype=dt_, device=dev, requires_grad=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, _ = rollout(r, v, mu, gam, steps, every)
+    loss = ((x - target) ** 2).sum(1).mean()
+    loss.backward()
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    its = r.iter_log
+    return {"workload": DIFF_WORKLOAD, "fluid_particles": nf, "steps": steps, "checkpoint_every": every,
+            "ms_per_step_fwd": round(t_fwd / steps * 1e3, 4), "ms_per_step_fwd_adj": round(t_adj / steps * 1e3, 4),
+            "ratio": round(t_adj / t_fwd, 3), "loss": float(loss.detach()),
+            "grad_mu": float(mu.grad), "grad_gamma": float(gam.grad),
+            "grad_v0_norm": float(v.grad.double().norm()),
+            "mean_iterations": [sum(a for a, _ in its) / len(its), sum(b for _, b in its) / len(its)]}
+
+
 def cpu_baseline(steps=2):
     """The oracle (C++ restatement of the reference step, reference schedule:
     atomic bucket grid each step, unsorted storage, separate (obj, nobj)
@@ -245,6 +301,8 @@ def main():
         del sim
         torch.cuda.empty_cache()
         out["adjoint"] = adjoint_leg(scen, args.adjoint_steps, args.checkpoint_every)
+        torch.cuda.empty_cache()
+        out["adjoint_" + DIFF_WORKLOAD] = differentiable_leg(scen)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

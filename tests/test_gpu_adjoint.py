@@ -108,6 +108,43 @@ def test_adjoint_matches_central_fd_fp64(name, steps, every):
     assert not bad, bad
 
 
+def test_adjoint_at_differentiable_dam_break_size_fp64():
+    """The 200k-particle differentiable dam break (target from mu = 0.02,
+    gamma = 0.05; start mu = 0.01, gamma = 0.07, v0 ~ N(0, 0.01^2)) over 20
+    steps checkpointed every 5, against FP64 central differences.
+
+    The v0 direction is the hardest case: measured distance of the FD
+    quotient to the adjoint for velocity steps 1e-8 / 1e-9 / 1e-10 / 1e-11 /
+    1e-12 is 4.2e-2 / 4.2e-3 / 8.5e-4 / 1.0e-2 / 1.0e-2 (psi-clamp kinks
+    above 1e-10; below it round-off, the loss being ~1e-12), so it uses
+    1e-10 and a bar of 2e-3; mu and gamma agree to ~5e-5."""
+    steps, every = 20, 5
+    r, st0, v0, target, rollout = setup_case("diff_3d_200k", steps)
+    mu = torch.tensor(0.01, dtype=torch.float64, device=v0.device, requires_grad=True)
+    gam = torch.tensor(0.07, dtype=torch.float64, device=v0.device, requires_grad=True)
+    v = v0.clone().requires_grad_(True)
+    x, _ = rollout(r, v, mu, gam, steps, every)
+    loss_of(x, target).backward()
+    base_log = list(r.iter_log)
+    print(f"  iterations {base_log}", flush=True)
+
+    def L_at(mu_, gam_, v_):
+        return float(loss_of(forward_only(r, st0, v_, mu_, gam_, steps, force=base_log), target))
+
+    g = torch.Generator().manual_seed(3)
+    dv = torch.randn(v0.shape, generator=g, dtype=torch.float64).to(v0.device)
+    checks = [("mu", float(mu.grad), (L_at(0.01 + 1e-6, 0.07, v0) - L_at(0.01 - 1e-6, 0.07, v0)) / 2e-6),
+              ("gamma", float(gam.grad), (L_at(0.01, 0.07 + 7e-6, v0) - L_at(0.01, 0.07 - 7e-6, v0)) / 1.4e-5),
+              ("v0 . dv", float((v.grad * dv).sum()),
+               (L_at(0.01, 0.07, v0 + 1e-10 * dv) - L_at(0.01, 0.07, v0 - 1e-10 * dv)) / 2e-10)]
+    bad = []
+    for what, ad, fd in checks:
+        print(f"  {what:8s} adjoint={ad:+.10e} fd={fd:+.10e} rel={rel(ad, fd):.2e}", flush=True)
+        if rel(ad, fd) > (2e-3 if what == "v0 . dv" else FD_TOL):
+            bad.append(what)
+    assert not bad, bad
+
+
 def test_position_adjoint_matches_fd_fp64():
     """d loss / d (initial positions) along a random direction: exercises
     every x-derivative (pass A, both solves, viscosity and normals)."""

@@ -102,6 +102,10 @@ def scenes():
                                    [([0, 0, 0], [16, 8, 16], [1, 0]), ([0, 8, 0], [16, 8, 16], [0, 1])],
                                    [1000, 500], [0, -9.81, 0], 0.01),
                               solver_config("VFSPH", cs=80, fbm_d=1e-4, fbm_c=50, gamma=0.07))
+    # Differentiable dam break: 50 x 80 x 50 = 200,000 fluid in a 1.0 x 1.0 x 0.5 m tank, r = 0.005,
+    # starting from mu = 0.01, gamma = 0.07 (the gradients are taken w.r.t. both)
+    out["diff_3d_200k"] = (tank(0.01, [100, 100, 50], 2, [([0, 0, 0], [50, 80, 50], [1])], [1000],
+                                [0, -9.81, 0], 0.01), solver_config("DFSPH", cs=100, gamma=0.07))
     return out
 
 
