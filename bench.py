@@ -195,6 +195,62 @@ def differentiable_leg(scen, steps=200, every=20):
             "mean_iterations": [sum(a for a, _ in its) / len(its), sum(b for _, b in its) / len(its)]}
 
 
+def batched_leg(steps=20, n_scenes=64):
+    """Batched dam breaks: `n_scenes` independent 3D DFSPH scenes (fluid
+    counts log-uniform in [10k, 200k], numpy default_rng(0)), advanced
+    sequentially (each Simulation's own sph_step) and as one batch (one
+    grouped launch per kernel, autotuned block counts), eager and replayed
+    from hipGraphs; fluid particle-steps/s of each."""
+    import torch
+    from tichi_sph_amd import simulation_from_dicts
+    from tichi_sph_amd.batch import SceneBatch
+    from tichi_sph_amd.scenarios.make_scenarios import batched_dambreaks
+
+    sims = [simulation_from_dicts(c, sc, cfl=False) for _, sc, c in batched_dambreaks(n_scenes, seed=0)]
+    nf = sum(s.fluid.part_num for s in sims)
+    names = ("pos", "vel", "mass", "rest_density", "rest_volume", "volume_frac", "pid")
+    cp = [({k: s.fluid.t[k][:s.fluid.part_num].clone() for k in names}, s.ctrl.clone()) for s in sims]
+
+    def restore():
+        for s, (d, c) in zip(sims, cp):
+            for k in names:
+                s.fluid.t[k][:s.fluid.part_num].copy_(d[k])
+            s.ctrl.copy_(c)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    for s in sims:  # warm-up (also sorts the boundaries)
+        s.sph_step()
+    restore()
+    t_seq = timed(lambda: [s.sph_step() for s in sims])
+    restore()
+    batch = SceneBatch(sims)
+    grid = batch.autotune()
+    batch.step()
+    restore()
+    t_batch = timed(batch.step)
+    restore()
+    batch.step()
+    batch.capture()
+    restore()
+    t_graph = timed(batch.step)
+    its = batch.iter_log[-1]
+    return {"workload": f"batched_dambreaks_{n_scenes}", "scenes": n_scenes, "fluid_particles": nf,
+            "steps": steps, "sequential_value": round(nf * steps / t_seq, 1),
+            "batched_value": round(nf * steps / t_batch, 1), "graph_value": round(nf * steps / t_graph, 1),
+            "ms_per_step_sequential": round(t_seq / steps * 1e3, 3),
+            "ms_per_step_batched": round(t_batch / steps * 1e3, 3),
+            "ms_per_step_graph": round(t_graph / steps * 1e3, 3),
+            "speedup_batched": round(t_seq / t_batch, 3), "speedup_graph": round(t_seq / t_graph, 3),
+            "autotuned_grid": grid, "max_iterations_last_step": [int(its[0].max()), int(its[1].max())]}
+
+
 def cpu_baseline(steps=2):
     """The oracle (C++ restatement of the reference step, reference schedule:
     atomic bucket grid each step, unsorted storage, separate (obj, nobj)
@@ -236,6 +292,8 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=20)
     ap.add_argument("--checkpoint-every", type=int, default=10)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus must be 1)",
@@ -303,6 +361,9 @@ def main():
         out["adjoint"] = adjoint_leg(scen, args.adjoint_steps, args.checkpoint_every)
         torch.cuda.empty_cache()
         out["adjoint_" + DIFF_WORKLOAD] = differentiable_leg(scen)
+    if not args.no_batched:
+        torch.cuda.empty_cache()
+        out["batched"] = batched_leg(args.batched_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -306,6 +306,56 @@ int tsph_adj_end(const TsphParams* p, TsphObject* fluid, TsphAdjoint* adj, const
                  void* xbar_ref, void* vbar_ref, void* stream);
 
 /* ----------------------------------------------------------------------
+ * Batched scenes (no reference counterpart: the reference runs one scene
+ * per process).  Many independent scenes of uneven particle counts are
+ * advanced together: every call below is ONE grouped launch per kernel over
+ * all scenes, driven by a device-resident scene table (constants, object
+ * descriptors, control block and workspace of each scene).  Each scene keeps
+ * its own stop rule; scenes whose solve has ended are predicated off on the
+ * device.  The table changes only on the device (the sort swaps each
+ * scene's front/back state pointers inside it), so a captured hipGraph of
+ * these calls stays valid across replays.  Particle counts must not change
+ * while a table is in use.  FBM is not batched.
+ * ---------------------------------------------------------------------- */
+typedef struct TsphBatchInfo {
+  int32_t nscenes, precision;
+  int32_t tiles[4]; /* tiles of a grouped launch: fluid, boundary, cell, scene */
+} TsphBatchInfo;
+
+/* Device bytes of the scene table for nscenes scenes. */
+size_t tsph_batch_table_bytes(int32_t precision, int32_t nscenes);
+
+/* Builds the scene table for scenes s = 0..nscenes-1 (params[s], fluid[s],
+ * bound[s], ctrl[s], workspace[s] of ws_bytes[s]) in `host_staging` (pinned
+ * host memory of table_bytes, which must stay untouched until the stream
+ * has executed the copy) and copies it to the device `table`.  Boundary
+ * objects must already be sorted.  Fills *info. */
+int tsph_batch_prepare(const TsphParams* params, const TsphObject* fluid, const TsphObject* bound,
+                       TsphControl* const* ctrl, void* const* workspace, const size_t* ws_bytes,
+                       int32_t nscenes, void* host_staging, void* table, size_t table_bytes,
+                       TsphBatchInfo* info, void* stream);
+
+/* tsph_sort_object for every scene's fluid.  Swaps the front/back pointers
+ * inside the device table and in the caller's host copies fluid[0..n-1]
+ * (pass NULL to leave host copies alone, e.g. when replaying a graph). */
+int tsph_batch_sort(const TsphBatchInfo* info, void* table, TsphObject* fluid, int32_t cfl,
+                    int32_t grid, void* stream);
+/* tsph_density_alpha, tsph_solver_begin, _psi, _update, tsph_advect and
+ * tsph_tail (fbm = 0) for every scene.  `grid` = number of blocks of the
+ * grouped launch (0: one block per tile); the result does not depend on it. */
+int tsph_batch_density_alpha(const TsphBatchInfo* info, void* table, int32_t grid, void* stream);
+int tsph_batch_solver_begin(const TsphBatchInfo* info, void* table, int32_t solve, void* stream);
+int tsph_batch_solver_psi(const TsphBatchInfo* info, void* table, int32_t k, int32_t grid,
+                          void* stream);
+int tsph_batch_solver_update(const TsphBatchInfo* info, void* table, int32_t k, int32_t grid,
+                             void* stream);
+int tsph_batch_advect(const TsphBatchInfo* info, void* table, int32_t grid, void* stream);
+int tsph_batch_tail(const TsphBatchInfo* info, void* table, int32_t grid, void* stream);
+/* Gathers every scene's (it, stop) of its current solve into the device
+ * int32 array out[2 * nscenes], for one copy to the host. */
+int tsph_batch_stop_state(const TsphBatchInfo* info, void* table, int32_t* out, void* stream);
+
+/* ----------------------------------------------------------------------
  * Reference kernel surface: the sph.py kernels one by one, with the
  * reference's (ngrid, obj, nobj) signature, walking the Ngrid built by
  * tsph_ngrid_build exactly as the reference does.  `kernel` is one of the

@@ -89,3 +89,19 @@ def test_build_recipe_flags():
     assert "--offload-arch=gfx950" in txt
     assert "-packed-fp32-ops" in txt
     assert "fast-math" not in txt
+
+
+def test_batch_table_sizing_and_argument_checks():
+    import ctypes
+    from tichi_sph_amd import _lib
+    lib = _lib.load()
+    b1, b64 = lib.tsph_batch_table_bytes(4, 1), lib.tsph_batch_table_bytes(4, 64)
+    assert 0 < b1 < b64 and lib.tsph_batch_table_bytes(8, 64) > b64
+    assert lib.tsph_batch_table_bytes(4, 0) == 0
+    info = _lib.TsphBatchInfo()
+    rc = lib.tsph_batch_prepare(None, None, None, None, None, None, 0, None, None, 0,
+                                ctypes.byref(info), None)
+    assert rc == -1 and b"null argument" in lib.tsph_last_error()
+    rc = lib.tsph_batch_density_alpha(ctypes.byref(info), None, 0, None)
+    assert rc == -1
+

@@ -78,6 +78,10 @@ class TsphAdjoint(ctypes.Structure):
     _fields_ = [(name, _vp) for name in _ADJ_PTRS]
 
 
+class TsphBatchInfo(ctypes.Structure):
+    _fields_ = [("nscenes", _i32), ("precision", _i32), ("tiles", _i32 * 4)]
+
+
 KERNEL_IDS = {
     "SPH_clean_value": 0, "cfl_condition": 1, "SPH_prepare_attr": 2,
     "SPH_prepare_alpha_1": 3, "SPH_prepare_alpha_2": 4, "SPH_prepare_alpha": 5,
@@ -97,6 +101,9 @@ EXPORTED = [
     "tsph_fbm_round", "tsph_tail",
     "tsph_ref_kernel", "tsph_solver_psi_taped", "tsph_relist", "tsph_adj_begin",
     "tsph_adj_solver_iteration", "tsph_adj_advect", "tsph_adj_density_alpha", "tsph_adj_end",
+    "tsph_batch_table_bytes", "tsph_batch_prepare", "tsph_batch_sort", "tsph_batch_density_alpha",
+    "tsph_batch_solver_begin", "tsph_batch_solver_psi", "tsph_batch_solver_update",
+    "tsph_batch_advect", "tsph_batch_tail", "tsph_batch_stop_state",
 ]
 
 _lib = None
@@ -127,6 +134,9 @@ def load(path=None):
     lib.tsph_workspace_bytes.argtypes = [ctypes.POINTER(TsphParams), _i32]
     P, O, C = ctypes.POINTER(TsphParams), ctypes.POINTER(TsphObject), _vp
     A = ctypes.POINTER(TsphAdjoint)
+    BI = ctypes.POINTER(TsphBatchInfo)
+    lib.tsph_batch_table_bytes.restype = ctypes.c_size_t
+    lib.tsph_batch_table_bytes.argtypes = [_i32, _i32]
     sz = ctypes.c_size_t
     sigs = {
         "tsph_sort_object": [P, O, C, _i32, _vp, sz, _vp],
@@ -148,6 +158,15 @@ def load(path=None):
         "tsph_adj_advect": [P, O, C, A, _vp, sz, _vp],
         "tsph_adj_density_alpha": [P, O, O, A, _vp],
         "tsph_adj_end": [P, O, A, _vp, _vp, _vp, _vp],
+        "tsph_batch_prepare": [P, O, O, _vp, _vp, _vp, _i32, _vp, _vp, sz, BI, _vp],
+        "tsph_batch_sort": [BI, _vp, O, _i32, _i32, _vp],
+        "tsph_batch_density_alpha": [BI, _vp, _i32, _vp],
+        "tsph_batch_solver_begin": [BI, _vp, _i32, _vp],
+        "tsph_batch_solver_psi": [BI, _vp, _i32, _i32, _vp],
+        "tsph_batch_solver_update": [BI, _vp, _i32, _i32, _vp],
+        "tsph_batch_advect": [BI, _vp, _i32, _vp],
+        "tsph_batch_tail": [BI, _vp, _i32, _vp],
+        "tsph_batch_stop_state": [BI, _vp, _vp, _vp],
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)

@@ -32,6 +32,60 @@ int solver_half(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, 
 template <class T>
 int relist(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template <class T>
+int batch_sort(const TsphBatchInfo&, char*, TsphObject*, int, int, hipStream_t);
+template <class T>
+int batch_density_alpha(const TsphBatchInfo&, char*, int, hipStream_t);
+template <class T>
+int batch_solver_begin(const TsphBatchInfo&, char*, int, hipStream_t);
+template <class T>
+int batch_solver_half(const TsphBatchInfo&, char*, int, int, int, hipStream_t);
+template <class T>
+int batch_advect(const TsphBatchInfo&, char*, int, hipStream_t);
+template <class T>
+int batch_tail(const TsphBatchInfo&, char*, int, hipStream_t);
+template <class T>
+int batch_stop_state(const TsphBatchInfo&, char*, int*, hipStream_t);
+
+template <class T>
+int batch_prepare(const TsphParams* p, const TsphObject* f, const TsphObject* b,
+                  TsphControl* const* ctrl, void* const* ws, const size_t* wsb, int ns,
+                  void* host, void* table, size_t bytes, TsphBatchInfo* info, hipStream_t st) {
+  TSPH_REQUIRE(bytes >= batch_table_bytes<T>(ns), TSPH_ECAPACITY,
+               "tsph_batch_prepare: table needs %zu bytes", batch_table_bytes<T>(ns));
+  std::memset(host, 0, batch_table_bytes<T>(ns));
+  SceneEntry<T>* E = (SceneEntry<T>*)host;
+  int* first = (int*)((char*)host + batch_entries_bytes<T>(ns));
+  int tot[kTileKinds] = {0, 0, 0, 0};
+  for (int s = 0; s < ns; ++s) {
+    TSPH_REQUIRE(p[s].precision == p[0].precision, TSPH_EINVAL,
+                 "tsph_batch_prepare: scene %d has another precision", s);
+    TSPH_REQUIRE(f[s].n > 0, TSPH_EINVAL, "tsph_batch_prepare: scene %d has no fluid", s);
+    TSPH_REQUIRE(p[s].list_cap_f >= 1 && p[s].list_cap_b >= 1, TSPH_EINVAL,
+                 "tsph_batch_prepare: list capacities must be positive");
+    E[s].k = make_kp<T>(p[s]);
+    E[s].f = f[s];
+    E[s].b = b[s];
+    E[s].ctrl = ctrl[s];
+    TSPH_REQUIRE(carve_ws(ws[s], wsb[s], f[s].capacity, p[s].node_num, &E[s].L), TSPH_ECAPACITY,
+                 "tsph_batch_prepare: workspace of scene %d too small", s);
+    const int t[kTileKinds] = {blocks_for(f[s].n), blocks_for(b[s].n),
+                               (int)cell_tiles(p[s].node_num), 1};
+    for (int q = 0; q < kTileKinds; ++q) {
+      first[q * (ns + 1) + s] = tot[q];
+      tot[q] += t[q];
+    }
+  }
+  for (int q = 0; q < kTileKinds; ++q) {
+    first[q * (ns + 1) + ns] = tot[q];
+    info->tiles[q] = tot[q];
+  }
+  info->nscenes = ns;
+  info->precision = p[0].precision;
+  hipError_t e = hipMemcpyAsync(table, host, batch_table_bytes<T>(ns), hipMemcpyHostToDevice, st);
+  TSPH_REQUIRE(e == hipSuccess, TSPH_EHIP, "tsph_batch_prepare: %s", hipGetErrorString(e));
+  return 0;
+}
+template <class T>
 int adj_begin(const TsphParams&, TsphObject*, TsphControl*, TsphAdjoint*, const int*, const void*,
               const void*, hipStream_t);
 template <class T>
@@ -299,6 +353,75 @@ int tsph_adj_density_alpha(const TsphParams* p, TsphObject* f, TsphObject* b, Ts
   if (int rc = check_obj(b, "tsph_adj_density_alpha")) return rc;
   if (int rc = check_adj(p, adj, "tsph_adj_density_alpha")) return rc;
   return DISPATCH(p, adj_density_alpha, *p, f, b, adj, (hipStream_t)stream);
+}
+
+size_t tsph_batch_table_bytes(int32_t precision, int32_t nscenes) {
+  if (nscenes < 1) return 0;
+  return precision == 8 ? batch_table_bytes<double>(nscenes) : batch_table_bytes<float>(nscenes);
+}
+
+int tsph_batch_prepare(const TsphParams* params, const TsphObject* fluid, const TsphObject* bound,
+                       TsphControl* const* ctrl, void* const* workspace, const size_t* ws_bytes,
+                       int32_t nscenes, void* host_staging, void* table, size_t table_bytes,
+                       TsphBatchInfo* info, void* stream) {
+  TSPH_REQUIRE(nscenes >= 1 && params && fluid && bound && ctrl && workspace && ws_bytes &&
+                   host_staging && table && info,
+               TSPH_EINVAL, "tsph_batch_prepare: null argument or no scenes");
+  for (int s = 0; s < nscenes; ++s) {
+    if (int rc = check_params(&params[s])) return rc;
+    if (int rc = check_obj(&fluid[s], "tsph_batch_prepare")) return rc;
+    if (int rc = check_obj(&bound[s], "tsph_batch_prepare")) return rc;
+  }
+  return DISPATCH(&params[0], batch_prepare, params, fluid, bound, ctrl, workspace, ws_bytes,
+                  nscenes, host_staging, table, table_bytes, info, (hipStream_t)stream);
+}
+
+static int check_batch(const TsphBatchInfo* bi, void* table, const char* what) {
+  TSPH_REQUIRE(bi && table && bi->nscenes >= 1, TSPH_EINVAL, "%s: bad batch", what);
+  TSPH_REQUIRE(bi->precision == 4 || bi->precision == 8, TSPH_EINVAL, "%s: bad precision", what);
+  return 0;
+}
+
+#define BDISPATCH(bi, fn, ...) \
+  ((bi)->precision == 8 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
+
+int tsph_batch_sort(const TsphBatchInfo* bi, void* table, TsphObject* fluid, int32_t cfl,
+                    int32_t grid, void* stream) {
+  if (int rc = check_batch(bi, table, "tsph_batch_sort")) return rc;
+  return BDISPATCH(bi, batch_sort, *bi, (char*)table, fluid, cfl, grid, (hipStream_t)stream);
+}
+int tsph_batch_density_alpha(const TsphBatchInfo* bi, void* table, int32_t grid, void* stream) {
+  if (int rc = check_batch(bi, table, "tsph_batch_density_alpha")) return rc;
+  return BDISPATCH(bi, batch_density_alpha, *bi, (char*)table, grid, (hipStream_t)stream);
+}
+int tsph_batch_solver_begin(const TsphBatchInfo* bi, void* table, int32_t solve, void* stream) {
+  if (int rc = check_batch(bi, table, "tsph_batch_solver_begin")) return rc;
+  return BDISPATCH(bi, batch_solver_begin, *bi, (char*)table, solve, (hipStream_t)stream);
+}
+int tsph_batch_solver_psi(const TsphBatchInfo* bi, void* table, int32_t k, int32_t grid,
+                          void* stream) {
+  if (int rc = check_batch(bi, table, "tsph_batch_solver_psi")) return rc;
+  TSPH_REQUIRE(k >= 0 && k < TSPH_MAX_ITER_LOG, TSPH_EINVAL, "tsph_batch_solver_psi: bad k");
+  return BDISPATCH(bi, batch_solver_half, *bi, (char*)table, k, 1, grid, (hipStream_t)stream);
+}
+int tsph_batch_solver_update(const TsphBatchInfo* bi, void* table, int32_t k, int32_t grid,
+                             void* stream) {
+  if (int rc = check_batch(bi, table, "tsph_batch_solver_update")) return rc;
+  TSPH_REQUIRE(k >= 0 && k < TSPH_MAX_ITER_LOG, TSPH_EINVAL, "tsph_batch_solver_update: bad k");
+  return BDISPATCH(bi, batch_solver_half, *bi, (char*)table, k, 2, grid, (hipStream_t)stream);
+}
+int tsph_batch_advect(const TsphBatchInfo* bi, void* table, int32_t grid, void* stream) {
+  if (int rc = check_batch(bi, table, "tsph_batch_advect")) return rc;
+  return BDISPATCH(bi, batch_advect, *bi, (char*)table, grid, (hipStream_t)stream);
+}
+int tsph_batch_tail(const TsphBatchInfo* bi, void* table, int32_t grid, void* stream) {
+  if (int rc = check_batch(bi, table, "tsph_batch_tail")) return rc;
+  return BDISPATCH(bi, batch_tail, *bi, (char*)table, grid, (hipStream_t)stream);
+}
+int tsph_batch_stop_state(const TsphBatchInfo* bi, void* table, int32_t* out, void* stream) {
+  if (int rc = check_batch(bi, table, "tsph_batch_stop_state")) return rc;
+  TSPH_REQUIRE(out, TSPH_EINVAL, "tsph_batch_stop_state: null out");
+  return BDISPATCH(bi, batch_stop_state, *bi, (char*)table, out, (hipStream_t)stream);
 }
 
 int tsph_ref_kernel(const TsphParams* p, int32_t kernel, TsphObject* obj, const TsphObject* nobj,

@@ -42,6 +42,12 @@ struct KP {
   T wg_a, wg_b;  // fast gradient factors: 6*sig/h^2 and 6*sig/h
 };
 
+// A kernel body's block: index and count within its launch, or within its
+// scene's share of a grouped (batched) launch.
+struct Blk {
+  int b, nb;
+};
+
 // ---------------------------------------------------------------------------
 // Block mapping.  Workgroups are dispatched round-robin over the 8 XCDs
 // (block b runs on XCD b % 8); remapping gives each XCD one contiguous range
@@ -262,8 +268,7 @@ __device__ __forceinline__ double min_partials(const double* part, int n, double
 enum FinalizeMode { kFinCfl = 0, kFinEnergy = 1, kFinFlag = 2, kFinCompression = 3 };
 
 template <class T>
-__global__ void __launch_bounds__(kBlock)
-    k_finalize(KP<T> k, int mode, const double* __restrict__ partial, int nblk, int n,
+__device__ __forceinline__ void k_finalize_body(Blk blk, KP<T> k, int mode, const double* __restrict__ partial, int nblk, int n,
                TsphControl* ctrl) {
   __shared__ double lds[kWaves];
   if (mode == kFinCfl) {
@@ -291,6 +296,13 @@ __global__ void __launch_bounds__(kBlock)
       ctrl->compression = (double)((T)a / (T)n);  // sph.py:226
     }
   }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_finalize(KP<T> k, int mode, const double* __restrict__ partial, int nblk, int n,
+               TsphControl* ctrl) {
+  k_finalize_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, mode, partial, nblk, n, ctrl);
 }
 
 }  // namespace tsph

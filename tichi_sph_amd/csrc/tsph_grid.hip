@@ -21,20 +21,26 @@ namespace tsph {
 // ---------------------------------------------------------------------------
 // Scan: exclusive prefix of `cnt[0..len)` into `start[0..len]`.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_scan_reduce(const int* __restrict__ cnt, long len,
+__device__ __forceinline__ void k_scan_reduce_body(Blk blk, const int* __restrict__ cnt, long len,
                                                         int* __restrict__ sums) {
   __shared__ double lds[kWaves];
-  long base = (long)blockIdx.x * kScanTile;
+  long base = (long)blk.b * kScanTile;
   long s = 0;
   for (int e = 0; e < 16; ++e) {
     long i = base + (long)e * kBlock + threadIdx.x;
     if (i < len) s += cnt[i];
   }
   double t = block_sum((double)s, lds);
-  if (threadIdx.x == 0) sums[blockIdx.x] = (int)t;
+  if (threadIdx.x == 0) sums[blk.b] = (int)t;
 }
 
-__global__ void __launch_bounds__(kBlock) k_scan_apply(const int* __restrict__ cnt, long len,
+__global__ void __launch_bounds__(kBlock)
+    k_scan_reduce(const int* __restrict__ cnt, long len,
+                                                        int* __restrict__ sums) {
+  k_scan_reduce_body(Blk{(int)blockIdx.x, (int)gridDim.x}, cnt, len, sums);
+}
+
+__device__ __forceinline__ void k_scan_apply_body(Blk blk, const int* __restrict__ cnt, long len,
                                                        const int* __restrict__ sums,
                                                        int* __restrict__ start) {
   __shared__ int wsum[kWaves];
@@ -42,12 +48,12 @@ __global__ void __launch_bounds__(kBlock) k_scan_apply(const int* __restrict__ c
   // offset of this tile = sum of previous tile sums
   if (threadIdx.x < 64) {
     int acc = 0;
-    for (int i = threadIdx.x; i < (int)blockIdx.x; i += 64) acc += sums[i];
+    for (int i = threadIdx.x; i < (int)blk.b; i += 64) acc += sums[i];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
     if (threadIdx.x == 0) boff = acc;
   }
-  long base = (long)blockIdx.x * kScanTile + (long)threadIdx.x * 16;
+  long base = (long)blk.b * kScanTile + (long)threadIdx.x * 16;
   int v[16];
   int loc = 0;
 #pragma unroll
@@ -79,6 +85,13 @@ __global__ void __launch_bounds__(kBlock) k_scan_apply(const int* __restrict__ c
   if (base <= len - 1 && len - 1 < base + 16) start[len] = run;
 }
 
+__global__ void __launch_bounds__(kBlock)
+    k_scan_apply(const int* __restrict__ cnt, long len,
+                                                       const int* __restrict__ sums,
+                                                       int* __restrict__ start) {
+  k_scan_apply_body(Blk{(int)blockIdx.x, (int)gridDim.x}, cnt, len, sums, start);
+}
+
 static int launch_scan(const int* cnt, long len, int* sums, int* start, hipStream_t st) {
   long nb = scan_blocks(len);
   if (nb < 1) nb = 1;
@@ -97,13 +110,12 @@ __global__ void __launch_bounds__(kBlock) k_fill_i32(int* __restrict__ p, long n
 // Fused-path sort.
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ void __launch_bounds__(kBlock)
-    k_keys(KP<T> k, const V4<T>* __restrict__ pos, const V4<T>* __restrict__ vel, int n,
+__device__ __forceinline__ void k_keys_body(Blk blk, KP<T> k, const V4<T>* __restrict__ pos, const V4<T>* __restrict__ vel, int n,
            int* __restrict__ key, int* __restrict__ rank, int* __restrict__ count, int cfl,
            double* __restrict__ partial, TsphControl* ctrl) {
   __shared__ double lds[kWaves];
-  int nb = gridDim.x;
-  int b = xcd_block(blockIdx.x, nb);
+  int nb = blk.nb;
+  int b = xcd_block(blk.b, nb);
   int i = b * kBlock + threadIdx.x;
   double cand = 1e300;
   const int lane = threadIdx.x & 63;
@@ -145,14 +157,21 @@ __global__ void __launch_bounds__(kBlock)
   }
   if (!cfl) return;
   double m = block_min(cand, lds);
-  if (threadIdx.x == 0) partial[blockIdx.x] = m;
+  if (threadIdx.x == 0) partial[blk.b] = m;
 }
 
+template <class T>
 __global__ void __launch_bounds__(kBlock)
-    k_scatter(const int* __restrict__ key, const int* __restrict__ rank,
+    k_keys(KP<T> k, const V4<T>* __restrict__ pos, const V4<T>* __restrict__ vel, int n,
+           int* __restrict__ key, int* __restrict__ rank, int* __restrict__ count, int cfl,
+           double* __restrict__ partial, TsphControl* ctrl) {
+  k_keys_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, pos, vel, n, key, rank, count, cfl, partial, ctrl);
+}
+
+__device__ __forceinline__ void k_scatter_body(Blk blk, const int* __restrict__ key, const int* __restrict__ rank,
               const int* __restrict__ start, const int* __restrict__ pid, int n,
               int* __restrict__ tmp_idx, int* __restrict__ tmp_pid, int* __restrict__ tmp_key) {
-  int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  int i = xcd_block(blk.b, blk.nb) * kBlock + threadIdx.x;
   if (i >= n) return;
   int kk = key[i];
   int s = start[kk] + rank[i];
@@ -161,9 +180,15 @@ __global__ void __launch_bounds__(kBlock)
   tmp_key[s] = kk;
 }
 
-template <class T>
 __global__ void __launch_bounds__(kBlock)
-    k_permute(int P, int n, const int* __restrict__ start, const int* __restrict__ tmp_idx,
+    k_scatter(const int* __restrict__ key, const int* __restrict__ rank,
+              const int* __restrict__ start, const int* __restrict__ pid, int n,
+              int* __restrict__ tmp_idx, int* __restrict__ tmp_pid, int* __restrict__ tmp_key) {
+  k_scatter_body(Blk{(int)blockIdx.x, (int)gridDim.x}, key, rank, start, pid, n, tmp_idx, tmp_pid, tmp_key);
+}
+
+template <class T>
+__device__ __forceinline__ void k_permute_body(Blk blk, int P, int n, const int* __restrict__ start, const int* __restrict__ tmp_idx,
               const int* __restrict__ tmp_pid, const int* __restrict__ tmp_key,
               const V4<T>* __restrict__ pos, const V4<T>* __restrict__ vel,
               const T* __restrict__ mass, const T* __restrict__ rdens, const T* __restrict__ rvol,
@@ -171,7 +196,7 @@ __global__ void __launch_bounds__(kBlock)
               V4<T>* __restrict__ vel_b, T* __restrict__ mass_b, T* __restrict__ rdens_b,
               T* __restrict__ rvol_b, T* __restrict__ vf_b, int* __restrict__ pid_b,
               int* __restrict__ cell) {
-  int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  int s = xcd_block(blk.b, blk.nb) * kBlock + threadIdx.x;
   if (s >= n) return;
   int c = tmp_key[s];
   int a = start[c], e = start[c + 1];
@@ -188,6 +213,29 @@ __global__ void __launch_bounds__(kBlock)
   for (int q = 0; q < P; ++q) vf_b[(long)dst * P + q] = vf[(long)src * P + q];
   pid_b[dst] = p;
   cell[dst] = c;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_permute(int P, int n, const int* __restrict__ start, const int* __restrict__ tmp_idx,
+              const int* __restrict__ tmp_pid, const int* __restrict__ tmp_key,
+              const V4<T>* __restrict__ pos, const V4<T>* __restrict__ vel,
+              const T* __restrict__ mass, const T* __restrict__ rdens, const T* __restrict__ rvol,
+              const T* __restrict__ vf, const int* __restrict__ pid, V4<T>* __restrict__ pos_b,
+              V4<T>* __restrict__ vel_b, T* __restrict__ mass_b, T* __restrict__ rdens_b,
+              T* __restrict__ rvol_b, T* __restrict__ vf_b, int* __restrict__ pid_b,
+              int* __restrict__ cell) {
+  k_permute_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, P, n, start, tmp_idx, tmp_pid, tmp_key, pos, vel, mass, rdens, rvol, vf, pid, pos_b, vel_b, mass_b, rdens_b, rvol_b, vf_b, pid_b, cell);
+}
+
+static void swap_state(TsphObject* o) {
+  std::swap(o->pos, o->pos_b);
+  std::swap(o->vel, o->vel_b);
+  std::swap(o->mass, o->mass_b);
+  std::swap(o->rest_density, o->rest_density_b);
+  std::swap(o->rest_volume, o->rest_volume_b);
+  std::swap(o->volume_frac, o->volume_frac_b);
+  std::swap(o->pid, o->pid_b);
 }
 
 template <class T>
@@ -223,13 +271,7 @@ int sort_object(const TsphParams& prm, TsphObject* o, TsphControl* ctrl, int cfl
                      (V4<T>*)o->pos_b, (V4<T>*)o->vel_b, (T*)o->mass_b, (T*)o->rest_density_b,
                      (T*)o->rest_volume_b, (T*)o->volume_frac_b, o->pid_b, o->cell);
   TSPH_LAUNCH_CHECK("permute");
-  std::swap(o->pos, o->pos_b);
-  std::swap(o->vel, o->vel_b);
-  std::swap(o->mass, o->mass_b);
-  std::swap(o->rest_density, o->rest_density_b);
-  std::swap(o->rest_volume, o->rest_volume_b);
-  std::swap(o->volume_frac, o->volume_frac_b);
-  std::swap(o->pid, o->pid_b);
+  swap_state(o);
   return 0;
 }
 
@@ -358,6 +400,117 @@ int ngrid_build(const TsphParams& prm, TsphObject* const* objs, int nobj, int* n
   TSPH_LAUNCH_CHECK("ngrid fill");
   return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Grouped (batched) sort: the kernels above over every scene of a table.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(kBlock) kb_fill_counts(char* tab, int ns) {
+  for_tiles<T, kTileCell>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
+    const long n = sort_len(e.k.node_num) + 1;  // cell_count[node_num + 2]
+    const long base = (long)blk.b * kScanTile;
+    for (int q = 0; q < 16; ++q) {
+      long i = base + (long)q * kBlock + threadIdx.x;
+      if (i < n) e.f.cell_count[i] = 0;
+    }
+  });
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) kb_keys(char* tab, int ns, int cfl) {
+  for_tiles<T, kTileFluid>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
+    k_keys_body<T>(blk, e.k, (const V4<T>*)e.f.pos, (const V4<T>*)e.f.vel, e.f.n, e.L.key, e.L.rank,
+                   e.f.cell_count, cfl, e.L.partial, e.ctrl);
+  });
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) kb_finalize_cfl(char* tab, int ns) {
+  const int* firstf = (const int*)(tab + batch_entries_bytes<T>(ns)) + kTileFluid * (ns + 1);
+  for_tiles<T, kTileScene>(tab, ns, [&](SceneEntry<T>& e, int s, Blk blk) {
+    k_finalize_body<T>(blk, e.k, (int)kFinCfl, e.L.partial, firstf[s + 1] - firstf[s], e.f.n,
+                       e.ctrl);
+  });
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) kb_scan_reduce(char* tab, int ns) {
+  for_tiles<T, kTileCell>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
+    k_scan_reduce_body(blk, e.f.cell_count, sort_len(e.k.node_num), e.L.scan_sums);
+  });
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) kb_scan_apply(char* tab, int ns) {
+  for_tiles<T, kTileCell>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
+    k_scan_apply_body(blk, e.f.cell_count, sort_len(e.k.node_num), e.L.scan_sums,
+                      e.f.cell_start);
+  });
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) kb_scatter(char* tab, int ns) {
+  for_tiles<T, kTileFluid>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
+    k_scatter_body(blk, e.L.key, e.L.rank, e.f.cell_start, e.f.pid, e.f.n, e.L.tmp_idx,
+                   e.L.tmp_pid, e.L.tmp_key);
+  });
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) kb_permute(char* tab, int ns) {
+  for_tiles<T, kTileFluid>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
+    const TsphObject& o = e.f;
+    k_permute_body<T>(blk, e.k.P, o.n, o.cell_start, e.L.tmp_idx, e.L.tmp_pid, e.L.tmp_key,
+                      (const V4<T>*)o.pos, (const V4<T>*)o.vel, (const T*)o.mass,
+                      (const T*)o.rest_density, (const T*)o.rest_volume,
+                      (const T*)o.volume_frac, o.pid, (V4<T>*)o.pos_b, (V4<T>*)o.vel_b,
+                      (T*)o.mass_b, (T*)o.rest_density_b, (T*)o.rest_volume_b,
+                      (T*)o.volume_frac_b, o.pid_b, o.cell);
+  });
+}
+
+// Front/back swap of every scene's persistent state inside the table.
+template <class T>
+__global__ void kb_swap(char* tab, int ns) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  TsphObject& o = ((SceneEntry<T>*)tab)[s].f;
+  void* t;
+#define TSPH_SWAP(a, b) t = (void*)o.a; o.a = decltype(o.a)(o.b); o.b = decltype(o.b)(t);
+  TSPH_SWAP(pos, pos_b)
+  TSPH_SWAP(vel, vel_b)
+  TSPH_SWAP(mass, mass_b)
+  TSPH_SWAP(rest_density, rest_density_b)
+  TSPH_SWAP(rest_volume, rest_volume_b)
+  TSPH_SWAP(volume_frac, volume_frac_b)
+  TSPH_SWAP(pid, pid_b)
+#undef TSPH_SWAP
+}
+
+template <class T>
+int batch_sort(const TsphBatchInfo& bi, char* tab, TsphObject* hostf, int cfl, int grid,
+               hipStream_t st) {
+  const int ns = bi.nscenes;
+  const int gf = grid_for(bi.tiles[kTileFluid], grid), gc = grid_for(bi.tiles[kTileCell], grid);
+  hipLaunchKernelGGL(kb_fill_counts<T>, dim3(gc), dim3(kBlock), 0, st, tab, ns);
+  if (gf) hipLaunchKernelGGL(kb_keys<T>, dim3(gf), dim3(kBlock), 0, st, tab, ns, cfl);
+  if (cfl)
+    hipLaunchKernelGGL(kb_finalize_cfl<T>, dim3(ns), dim3(kBlock), 0, st, tab, ns);
+  hipLaunchKernelGGL(kb_scan_reduce<T>, dim3(gc), dim3(kBlock), 0, st, tab, ns);
+  hipLaunchKernelGGL(kb_scan_apply<T>, dim3(gc), dim3(kBlock), 0, st, tab, ns);
+  if (gf) {
+    hipLaunchKernelGGL(kb_scatter<T>, dim3(gf), dim3(kBlock), 0, st, tab, ns);
+    hipLaunchKernelGGL(kb_permute<T>, dim3(gf), dim3(kBlock), 0, st, tab, ns);
+  }
+  hipLaunchKernelGGL(kb_swap<T>, dim3((ns + 63) / 64), dim3(64), 0, st, tab, ns);
+  TSPH_LAUNCH_CHECK("batch_sort");
+  if (hostf)
+    for (int s = 0; s < ns; ++s) swap_state(&hostf[s]);
+  return 0;
+}
+
+template int batch_sort<float>(const TsphBatchInfo&, char*, TsphObject*, int, int, hipStream_t);
+template int batch_sort<double>(const TsphBatchInfo&, char*, TsphObject*, int, int, hipStream_t);
 
 template int sort_object<float>(const TsphParams&, TsphObject*, TsphControl*, int, void*, size_t, hipStream_t);
 template int sort_object<double>(const TsphParams&, TsphObject*, TsphControl*, int, void*, size_t, hipStream_t);

@@ -106,12 +106,12 @@ struct Carve {
   }
 };
 
-inline size_t seg(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
+__host__ __device__ inline size_t seg(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
 
 // Scan tile: counts per scan block.
 constexpr int kScanTile = kBlock * 16;
 
-inline long scan_blocks(long len) { return (len + kScanTile - 1) / kScanTile; }
+__host__ __device__ inline long scan_blocks(long len) { return (len + kScanTile - 1) / kScanTile; }
 
 // Workspace layout used by all calls for objects of up to `cap` slots and a
 // grid of node_num cells (reference grid: node_num * 2 (cell,uid) buckets).
@@ -149,6 +149,66 @@ inline bool carve_ws(void* ws, size_t bytes, long cap, long node_num, WsLayout* 
   L->cu_count = c.take<int>(lenref + 1);
   L->cu_start = c.take<int>(lenref + 2);
   return c.ok;
+}
+
+// ---------------------------------------------------------------------------
+// Batched scenes.  A device-resident table holds one entry per scene (its
+// constants, object descriptors, control block and workspace layout),
+// followed by per-kind tile prefixes: first[kind][s] is the first tile of
+// scene s in a grouped launch of that kind.  Grouped kernels loop over tiles
+// (grid-stride), find each tile's scene by binary search and run the same
+// per-block body as the single-scene kernel.
+// ---------------------------------------------------------------------------
+template <class T>
+struct SceneEntry {
+  KP<T> k;
+  TsphObject f, b;
+  TsphControl* ctrl;
+  WsLayout L;
+};
+
+enum TileKind { kTileFluid = 0, kTileBound = 1, kTileCell = 2, kTileScene = 3, kTileKinds = 4 };
+
+template <class T>
+__host__ __device__ inline size_t batch_entries_bytes(int ns) {
+  return seg(sizeof(SceneEntry<T>) * (size_t)ns);
+}
+template <class T>
+__host__ __device__ inline size_t batch_table_bytes(int ns) {
+  return batch_entries_bytes<T>(ns) + sizeof(int) * kTileKinds * (size_t)(ns + 1);
+}
+
+// Cells of a scene's counting sort (cells + the trailing invalid bucket) and
+// the scan tiles covering them plus the count array's extra slot.
+__host__ __device__ inline long sort_len(long node_num) { return node_num + 1; }
+__host__ __device__ inline long cell_tiles(long node_num) { return scan_blocks(sort_len(node_num) + 1); }
+
+__device__ __forceinline__ int scene_of(const int* first, int ns, int t) {
+  int lo = 0, hi = ns - 1;  // largest s with first[s] <= t
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (first[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Grid-stride loop over the tiles of one kind; f(entry, scene, blk).
+template <class T, int KIND, class F>
+__device__ __forceinline__ void for_tiles(char* tab, int ns, F&& f) {
+  SceneEntry<T>* E = (SceneEntry<T>*)tab;
+  const int* first = (const int*)(tab + batch_entries_bytes<T>(ns)) + KIND * (ns + 1);
+  const int total = first[ns];
+  for (int t = blockIdx.x; t < total; t += gridDim.x) {
+    const int s = scene_of(first, ns, t);
+    const int f0 = first[s];
+    f(E[s], s, Blk{t - f0, first[s + 1] - f0});
+  }
+}
+
+inline int grid_for(int tiles, int grid) {
+  if (tiles <= 0) return 0;
+  return (grid <= 0 || grid > tiles) ? tiles : grid;
 }
 
 }  // namespace tsph

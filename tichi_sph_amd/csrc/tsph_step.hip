@@ -79,8 +79,7 @@ __device__ __forceinline__ void pass_a_epilogue(const KP<T>& k, int i, const V4<
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock)
-    k_pass_a_fluid(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
+__device__ __forceinline__ void k_pass_a_fluid_body(Blk blk, KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
                    const T* __restrict__ fmass, const T* __restrict__ frvol,
                    const int* __restrict__ fstart, int nb, const V4<T>* __restrict__ bpos,
                    const T* __restrict__ bmass, const T* __restrict__ brvol,
@@ -90,7 +89,7 @@ __global__ void __launch_bounds__(kBlock)
                    V4<T>* __restrict__ vel_adv, int* __restrict__ nbr_f, int* __restrict__ nbr_b,
                    int* __restrict__ nbr_nf, int* __restrict__ nbr_nb, int* __restrict__ flag,
                    TsphControl* ctrl) {
-  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int i = xcd_block(blk.b, blk.nb) * kBlock + threadIdx.x;
   if (i >= n) return;
   const int capf = k.list_cap_f, capb = k.list_cap_b;
   flag[i] = 0;  // SPH_clean_value, sph.py:30
@@ -120,11 +119,25 @@ __global__ void __launch_bounds__(kBlock)
                   vel_adv, nbr_nf, nbr_nb, ctrl);
 }
 
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_pass_a_fluid(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
+                   const T* __restrict__ fmass, const T* __restrict__ frvol,
+                   const int* __restrict__ fstart, int nb, const V4<T>* __restrict__ bpos,
+                   const T* __restrict__ bmass, const T* __restrict__ brvol,
+                   const int* __restrict__ bstart, T* __restrict__ W, T* __restrict__ comp,
+                   T* __restrict__ dens, V4<T>* __restrict__ alpha_1, T* __restrict__ alpha_2,
+                   T* __restrict__ alpha, T* __restrict__ vol, const V4<T>* __restrict__ vel,
+                   V4<T>* __restrict__ vel_adv, int* __restrict__ nbr_f, int* __restrict__ nbr_b,
+                   int* __restrict__ nbr_nf, int* __restrict__ nbr_nb, int* __restrict__ flag,
+                   TsphControl* ctrl) {
+  k_pass_a_fluid_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, n, cap, fpos, fmass, frvol, fstart, nb, bpos, bmass, brvol, bstart, W, comp, dens, alpha_1, alpha_2, alpha, vol, vel, vel_adv, nbr_f, nbr_b, nbr_nf, nbr_nb, flag, ctrl);
+}
+
 // Pass A, boundary particles: SPH_prepare_attr (b,b)(b,f), SPH_prepare_alpha_2
 // (b,f), SPH_prepare_alpha (alpha_1 of the boundary is never accumulated).
 template <class T>
-__global__ void __launch_bounds__(kBlock)
-    k_pass_a_bound(KP<T> k, int nb, const V4<T>* __restrict__ bpos, const T* __restrict__ bmass,
+__device__ __forceinline__ void k_pass_a_bound_body(Blk blk, KP<T> k, int nb, const V4<T>* __restrict__ bpos, const T* __restrict__ bmass,
                    const T* __restrict__ brvol, const int* __restrict__ bstart,
                    const V4<T>* __restrict__ fpos, const T* __restrict__ fmass,
                    const T* __restrict__ frvol, const int* __restrict__ fstart,
@@ -132,7 +145,7 @@ __global__ void __launch_bounds__(kBlock)
                    V4<T>* __restrict__ alpha_1, T* __restrict__ alpha_2, T* __restrict__ alpha,
                    T* __restrict__ vol) {
   TSPH_STRICT
-  int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  int i = xcd_block(blk.b, blk.nb) * kBlock + threadIdx.x;
   if (i >= nb) return;
   V4<T> p = bpos[i];
   int c[3];
@@ -177,9 +190,21 @@ __global__ void __launch_bounds__(kBlock)
   vol[i] = p.w / psi;
 }
 
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_pass_a_bound(KP<T> k, int nb, const V4<T>* __restrict__ bpos, const T* __restrict__ bmass,
+                   const T* __restrict__ brvol, const int* __restrict__ bstart,
+                   const V4<T>* __restrict__ fpos, const T* __restrict__ fmass,
+                   const T* __restrict__ frvol, const int* __restrict__ fstart,
+                   T* __restrict__ W, T* __restrict__ comp, T* __restrict__ dens,
+                   V4<T>* __restrict__ alpha_1, T* __restrict__ alpha_2, T* __restrict__ alpha,
+                   T* __restrict__ vol) {
+  k_pass_a_bound_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, nb, bpos, bmass, brvol, bstart, fpos, fmass, frvol, fstart, W, comp, dens, alpha_1, alpha_2, alpha, vol);
+}
+
 // Per-step reset of the control block's step-scoped state: general_flag = 1
 // (SPH_clean_value, sph.py:24), FBM round count, list-overflow counters.
-__global__ void k_step_reset(TsphControl* ctrl) {
+__device__ __forceinline__ void step_reset_body(TsphControl* ctrl) {
   if (threadIdx.x == 0) {
     ctrl->general_flag = 1;
     ctrl->fbm_rounds = 0;
@@ -187,12 +212,13 @@ __global__ void k_step_reset(TsphControl* ctrl) {
     ctrl->overflow_b = 0;
   }
 }
+__global__ void k_step_reset(TsphControl* ctrl) { step_reset_body(ctrl); }
 
 // ---------------------------------------------------------------------------
 // Solver stop state (main.py:201-213 / 222-234).
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ void k_solver_begin(KP<T> k, TsphControl* ctrl, int solve) {
+__device__ __forceinline__ void solver_begin_body(const KP<T>& k, TsphControl* ctrl, int solve) {
   int t = threadIdx.x;
   T thr = solve == 0 ? k.div_thr : k.comp_thr;
   T comp = (T)ctrl->compression;
@@ -207,12 +233,15 @@ __global__ void k_solver_begin(KP<T> k, TsphControl* ctrl, int solve) {
     else ctrl->incom_iter = 0;
   }
 }
+template <class T>
+__global__ void k_solver_begin(KP<T> k, TsphControl* ctrl, int solve) {
+  solver_begin_body(k, ctrl, solve);
+}
 
 // Jacobi half-step P: IPPE_adv_psi_init + IPPE_adv_psi (f,f)(f,b) +
 // IPPE_psi_adv_non_negative (sph.py:196-226) for iteration kk of the solve.
 template <class T>
-__global__ void __launch_bounds__(kBlock)
-    k_solver_psi(KP<T> k, int n, int cap, int kk, const V4<T>* __restrict__ fpos,
+__device__ __forceinline__ void k_solver_psi_body(Blk blk, KP<T> k, int n, int cap, int kk, const V4<T>* __restrict__ fpos,
                  const V4<T>* __restrict__ vadv, const T* __restrict__ psi_rho,
                  const T* __restrict__ rdens, const T* __restrict__ alpha,
                  const int* __restrict__ fstart, const V4<T>* __restrict__ bpos,
@@ -223,7 +252,7 @@ __global__ void __launch_bounds__(kBlock)
                  TsphControl* ctrl) {
   __shared__ double lds[kWaves];
   if (!ctrl->run_u[kk]) return;
-  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int i = xcd_block(blk.b, blk.nb) * kBlock + threadIdx.x;
   double err = 0;
   if (i < n) {
     const T dt = (T)ctrl->dt;
@@ -256,15 +285,28 @@ __global__ void __launch_bounds__(kBlock)
     err = (double)(psi / rp);
   }
   double s = block_sum(err, lds);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  if (threadIdx.x == 0) partial[blk.b] = s;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_solver_psi(KP<T> k, int n, int cap, int kk, const V4<T>* __restrict__ fpos,
+                 const V4<T>* __restrict__ vadv, const T* __restrict__ psi_rho,
+                 const T* __restrict__ rdens, const T* __restrict__ alpha,
+                 const int* __restrict__ fstart, const V4<T>* __restrict__ bpos,
+                 const int* __restrict__ bstart, const int* __restrict__ nbr_f,
+                 const int* __restrict__ nbr_b, const int* __restrict__ nbr_nf,
+                 const int* __restrict__ nbr_nb, T* __restrict__ psi_adv, T* __restrict__ kappa,
+                 V4<T>* __restrict__ tape_v, T* __restrict__ tape_k, double* __restrict__ partial,
+                 TsphControl* ctrl) {
+  k_solver_psi_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, n, cap, kk, fpos, vadv, psi_rho, rdens, alpha, fstart, bpos, bstart, nbr_f, nbr_b, nbr_nf, nbr_nb, psi_adv, kappa, tape_v, tape_k, partial, ctrl);
 }
 
 // Jacobi half-step U: IPPE_update_vel_adv (f,f)(f,b) (sph.py:229-244), with
 // kappa = psi_adv / alpha formed once per particle.  Block 0 also finishes
 // iteration kk's compression and takes the loop decision for iteration kk+1.
 template <class T>
-__global__ void __launch_bounds__(kBlock)
-    k_solver_update(KP<T> k, int n, int cap, int kk, const V4<T>* __restrict__ fpos,
+__device__ __forceinline__ void k_solver_update_body(Blk blk, KP<T> k, int n, int cap, int kk, const V4<T>* __restrict__ fpos,
                     const T* __restrict__ fmass, const T* __restrict__ kappa,
                     const int* __restrict__ fstart, const V4<T>* __restrict__ bpos,
                     const int* __restrict__ bstart, const int* __restrict__ nbr_f,
@@ -273,10 +315,10 @@ __global__ void __launch_bounds__(kBlock)
                     const double* __restrict__ partial, TsphControl* ctrl) {
   __shared__ double lds[kWaves];
   if (!ctrl->run_u[kk]) return;
-  if (blockIdx.x == 0) {
+  if (blk.b == 0) {
     // IPPE_psi_adv_non_negative's 0-D reduction (sph.py:221-226) and the loop
     // test of main.py:201-213 / 222-234
-    double tot = sum_partials(partial, gridDim.x, lds);
+    double tot = sum_partials(partial, blk.nb, lds);
     if (threadIdx.x == 0) {
       TSPH_STRICT
       T comp = (T)tot / (T)n;
@@ -295,7 +337,7 @@ __global__ void __launch_bounds__(kBlock)
       ctrl->stop = run ? 0 : 1;
     }
   }
-  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int i = xcd_block(blk.b, blk.nb) * kBlock + threadIdx.x;
   if (i >= n) return;
   const T dt = (T)ctrl->dt;
   const V4<T> p = fpos[i];
@@ -329,6 +371,18 @@ __global__ void __launch_bounds__(kBlock)
   vadv[i] = va;
 }
 
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_solver_update(KP<T> k, int n, int cap, int kk, const V4<T>* __restrict__ fpos,
+                    const T* __restrict__ fmass, const T* __restrict__ kappa,
+                    const int* __restrict__ fstart, const V4<T>* __restrict__ bpos,
+                    const int* __restrict__ bstart, const int* __restrict__ nbr_f,
+                    const int* __restrict__ nbr_b, const int* __restrict__ nbr_nf,
+                    const int* __restrict__ nbr_nb, V4<T>* __restrict__ vadv,
+                    const double* __restrict__ partial, TsphControl* ctrl) {
+  k_solver_update_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, n, cap, kk, fpos, fmass, kappa, fstart, bpos, bstart, nbr_f, nbr_b, nbr_nf, nbr_nb, vadv, partial, ctrl);
+}
+
 // ---------------------------------------------------------------------------
 // Advection: SPH_vel_adv_2_vel, gravity, viscosity (f,f), surface-tension
 // normals (f,f).  Neighbour velocities are read from vel_adv, so this sweep
@@ -336,14 +390,13 @@ __global__ void __launch_bounds__(kBlock)
 // the next kernel (k_surface_tension).
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ void __launch_bounds__(kBlock)
-    k_advect(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos, const T* __restrict__ vol,
+__device__ __forceinline__ void k_advect_body(Blk blk, KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos, const T* __restrict__ vol,
              const T* __restrict__ rdens, const int* __restrict__ fstart,
              const int* __restrict__ bstart, const V4<T>* __restrict__ bpos,
              const int* __restrict__ nbr_f, const int* __restrict__ nbr_nf,
              V4<T>* __restrict__ vadv, V4<T>* __restrict__ vel, V4<T>* __restrict__ acce_adv,
              V4<T>* __restrict__ normal) {
-  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int i = xcd_block(blk.b, blk.nb) * kBlock + threadIdx.x;
   if (i >= n) return;
   const V4<T> p = fpos[i];
   const V4<T> vi = vadv[i];
@@ -381,20 +434,30 @@ __global__ void __launch_bounds__(kBlock)
   normal[i] = V4<T>{nx * k.h, ny * k.h, nz * k.h, T(0)};
 }
 
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_advect(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos, const T* __restrict__ vol,
+             const T* __restrict__ rdens, const int* __restrict__ fstart,
+             const int* __restrict__ bstart, const V4<T>* __restrict__ bpos,
+             const int* __restrict__ nbr_f, const int* __restrict__ nbr_nf,
+             V4<T>* __restrict__ vadv, V4<T>* __restrict__ vel, V4<T>* __restrict__ acce_adv,
+             V4<T>* __restrict__ normal) {
+  k_advect_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, n, cap, fpos, vol, rdens, fstart, bstart, bpos, nbr_f, nbr_nf, vadv, vel, acce_adv, normal);
+}
+
 // Surface tension loop 2 (sph.py:151-166) + SPH_advection_update_vel_adv
 // (sph.py:247-250).  The curvature term carries no kernel weight, so every
 // candidate of the neighbouring cells contributes: this sweep walks the cells.
 // With gamma == 0 every term is an exact zero and only the update runs.
 template <class T>
-__global__ void __launch_bounds__(kBlock)
-    k_surface_tension(KP<T> k, int n, const V4<T>* __restrict__ fpos,
+__device__ __forceinline__ void k_surface_tension_body(Blk blk, KP<T> k, int n, const V4<T>* __restrict__ fpos,
                       const T* __restrict__ fmass, const T* __restrict__ psi_rho,
                       const T* __restrict__ rdens, const T* __restrict__ vf,
                       const int* __restrict__ fstart, const V4<T>* __restrict__ normal,
                       const V4<T>* __restrict__ vel, V4<T>* __restrict__ acce_adv,
                       V4<T>* __restrict__ vadv, const TsphControl* ctrl) {
   TSPH_STRICT
-  int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  int i = xcd_block(blk.b, blk.nb) * kBlock + threadIdx.x;
   if (i >= n) return;
   const T dt = (T)ctrl->dt;
   V4<T> a = acce_adv[i];
@@ -422,6 +485,17 @@ __global__ void __launch_bounds__(kBlock)
   }
   V4<T> v = vel[i];
   vadv[i] = V4<T>{v.x + a.x * dt, v.y + a.y * dt, v.z + a.z * dt, T(0)};
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_surface_tension(KP<T> k, int n, const V4<T>* __restrict__ fpos,
+                      const T* __restrict__ fmass, const T* __restrict__ psi_rho,
+                      const T* __restrict__ rdens, const T* __restrict__ vf,
+                      const int* __restrict__ fstart, const V4<T>* __restrict__ normal,
+                      const V4<T>* __restrict__ vel, V4<T>* __restrict__ acce_adv,
+                      V4<T>* __restrict__ vadv, const TsphControl* ctrl) {
+  k_surface_tension_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, n, fpos, fmass, psi_rho, rdens, vf, fstart, normal, vel, acce_adv, vadv, ctrl);
 }
 
 // ---------------------------------------------------------------------------
@@ -565,15 +639,14 @@ __global__ void __launch_bounds__(kBlock)
 // SPH_update_energy (sph.py:265-286, 375-379).
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ void __launch_bounds__(kBlock)
-    k_tail(KP<T> k, int n, int fbm, V4<T>* __restrict__ pos, V4<T>* __restrict__ vel,
+__device__ __forceinline__ void k_tail_body(Blk blk, KP<T> k, int n, int fbm, V4<T>* __restrict__ pos, V4<T>* __restrict__ vel,
            const V4<T>* __restrict__ vadv, T* __restrict__ mass, T* __restrict__ rdens,
            const T* __restrict__ rvol, T* __restrict__ vf, const T* __restrict__ tmp,
            const int* __restrict__ flag, double* __restrict__ partial, TsphControl* ctrl) {
   TSPH_STRICT
   __shared__ double lds[kWaves];
-  int nblk = gridDim.x;
-  int i = xcd_block(blockIdx.x, nblk) * kBlock + threadIdx.x;
+  int nblk = blk.nb;
+  int i = xcd_block(blk.b, nblk) * kBlock + threadIdx.x;
   const T dt = (T)ctrl->dt;
   double ke = 0, pe = 0;
   if (i < n) {
@@ -600,9 +673,18 @@ __global__ void __launch_bounds__(kBlock)
   double s1 = block_sum(ke, lds);
   double s2 = block_sum(pe, lds);
   if (threadIdx.x == 0) {
-    partial[blockIdx.x] = s1;
-    partial[nblk + blockIdx.x] = s2;
+    partial[blk.b] = s1;
+    partial[nblk + blk.b] = s2;
   }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_tail(KP<T> k, int n, int fbm, V4<T>* __restrict__ pos, V4<T>* __restrict__ vel,
+           const V4<T>* __restrict__ vadv, T* __restrict__ mass, T* __restrict__ rdens,
+           const T* __restrict__ rvol, T* __restrict__ vf, const T* __restrict__ tmp,
+           const int* __restrict__ flag, double* __restrict__ partial, TsphControl* ctrl) {
+  k_tail_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, n, fbm, pos, vel, vadv, mass, rdens, rvol, vf, tmp, flag, partial, ctrl);
 }
 
 // ---------------------------------------------------------------------------
@@ -750,6 +832,197 @@ int tail(const TsphParams& prm, TsphObject* f, TsphControl* ctrl, int fbm, void*
   TSPH_LAUNCH_CHECK("tail");
   return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Grouped (batched) launches: the kernels above over every scene of a table
+// (tsph_host.h:SceneEntry), with the single-scene launchers' arguments.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ const T* psi_of(const SceneEntry<T>& e) {
+  return (const T*)(e.k.solver == TSPH_VFSPH ? e.f.sph_compression : e.f.sph_density);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) kb_step_reset(char* tab, int ns) {
+  for_tiles<T, kTileScene>(tab, ns, [&](SceneEntry<T>& e, int, Blk) { step_reset_body(e.ctrl); });
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) kb_pass_a_fluid(char* tab, int ns) {
+  for_tiles<T, kTileFluid>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
+    TsphObject& f = e.f;
+    const TsphObject& b = e.b;
+    k_pass_a_fluid_body<T>(blk, e.k, f.n, f.capacity, (const V4<T>*)f.pos, (const T*)f.mass,
+                           (const T*)f.rest_volume, f.cell_start, b.n, (const V4<T>*)b.pos,
+                           (const T*)b.mass, (const T*)b.rest_volume, b.cell_start, (T*)f.W,
+                           (T*)f.sph_compression, (T*)f.sph_density, (V4<T>*)f.alpha_1,
+                           (T*)f.alpha_2, (T*)f.alpha, (T*)f.vol, (const V4<T>*)f.vel,
+                           (V4<T>*)f.vel_adv, f.nbr_f, f.nbr_b, f.nbr_nf, f.nbr_nb, f.flag,
+                           e.ctrl);
+  });
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) kb_pass_a_bound(char* tab, int ns) {
+  for_tiles<T, kTileBound>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
+    const TsphObject& f = e.f;
+    TsphObject& b = e.b;
+    k_pass_a_bound_body<T>(blk, e.k, b.n, (const V4<T>*)b.pos, (const T*)b.mass,
+                           (const T*)b.rest_volume, b.cell_start, (const V4<T>*)f.pos,
+                           (const T*)f.mass, (const T*)f.rest_volume, f.cell_start, (T*)b.W,
+                           (T*)b.sph_compression, (T*)b.sph_density, (V4<T>*)b.alpha_1,
+                           (T*)b.alpha_2, (T*)b.alpha, (T*)b.vol);
+  });
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) kb_solver_begin(char* tab, int ns, int solve) {
+  for_tiles<T, kTileScene>(tab, ns,
+                           [&](SceneEntry<T>& e, int, Blk) { solver_begin_body(e.k, e.ctrl, solve); });
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) kb_solver_psi(char* tab, int ns, int kk) {
+  for_tiles<T, kTileFluid>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
+    TsphObject& f = e.f;
+    k_solver_psi_body<T>(blk, e.k, f.n, f.capacity, kk, (const V4<T>*)f.pos,
+                         (const V4<T>*)f.vel_adv, psi_of(e), (const T*)f.rest_density,
+                         (const T*)f.alpha, f.cell_start, (const V4<T>*)e.b.pos, e.b.cell_start,
+                         f.nbr_f, f.nbr_b, f.nbr_nf, f.nbr_nb, (T*)f.psi_adv, (T*)f.kappa,
+                         (V4<T>*)nullptr, (T*)nullptr, e.L.partial, e.ctrl);
+  });
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) kb_solver_update(char* tab, int ns, int kk) {
+  for_tiles<T, kTileFluid>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
+    TsphObject& f = e.f;
+    k_solver_update_body<T>(blk, e.k, f.n, f.capacity, kk, (const V4<T>*)f.pos,
+                            (const T*)f.mass, (const T*)f.kappa, f.cell_start,
+                            (const V4<T>*)e.b.pos, e.b.cell_start, f.nbr_f, f.nbr_b, f.nbr_nf,
+                            f.nbr_nb, (V4<T>*)f.vel_adv, e.L.partial, e.ctrl);
+  });
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) kb_advect(char* tab, int ns) {
+  for_tiles<T, kTileFluid>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
+    TsphObject& f = e.f;
+    k_advect_body<T>(blk, e.k, f.n, f.capacity, (const V4<T>*)f.pos, (const T*)f.vol,
+                     (const T*)f.rest_density, f.cell_start, e.b.cell_start,
+                     (const V4<T>*)e.b.pos, f.nbr_f, f.nbr_nf, (V4<T>*)f.vel_adv, (V4<T>*)f.vel,
+                     (V4<T>*)f.acce_adv, (V4<T>*)f.normal);
+  });
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) kb_surface_tension(char* tab, int ns) {
+  for_tiles<T, kTileFluid>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
+    TsphObject& f = e.f;
+    k_surface_tension_body<T>(blk, e.k, f.n, (const V4<T>*)f.pos, (const T*)f.mass, psi_of(e),
+                              (const T*)f.rest_density, (const T*)f.volume_frac, f.cell_start,
+                              (const V4<T>*)f.normal, (const V4<T>*)f.vel, (V4<T>*)f.acce_adv,
+                              (V4<T>*)f.vel_adv, e.ctrl);
+  });
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) kb_tail(char* tab, int ns) {
+  for_tiles<T, kTileFluid>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
+    TsphObject& f = e.f;
+    k_tail_body<T>(blk, e.k, f.n, 0, (V4<T>*)f.pos, (V4<T>*)f.vel, (const V4<T>*)f.vel_adv,
+                   (T*)f.mass, (T*)f.rest_density, (const T*)f.rest_volume, (T*)f.volume_frac,
+                   (const T*)f.volume_frac_tmp, f.flag, e.L.partial, e.ctrl);
+  });
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) kb_finalize_energy(char* tab, int ns) {
+  const int* firstf = (const int*)(tab + batch_entries_bytes<T>(ns)) + kTileFluid * (ns + 1);
+  for_tiles<T, kTileScene>(tab, ns, [&](SceneEntry<T>& e, int s, Blk blk) {
+    k_finalize_body<T>(blk, e.k, (int)kFinEnergy, e.L.partial, firstf[s + 1] - firstf[s], e.f.n,
+                       e.ctrl);
+  });
+}
+
+template <class T>
+__global__ void kb_stop_state(char* tab, int ns, int* out) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const TsphControl* c = ((SceneEntry<T>*)tab)[s].ctrl;
+  out[2 * s] = c->it;
+  out[2 * s + 1] = c->stop;
+}
+
+template <class T>
+int batch_density_alpha(const TsphBatchInfo& bi, char* tab, int grid, hipStream_t st) {
+  const int ns = bi.nscenes;
+  hipLaunchKernelGGL(kb_step_reset<T>, dim3(ns), dim3(kBlock), 0, st, tab, ns);
+  if (int g = grid_for(bi.tiles[kTileFluid], grid))
+    hipLaunchKernelGGL(kb_pass_a_fluid<T>, dim3(g), dim3(kBlock), 0, st, tab, ns);
+  if (int g = grid_for(bi.tiles[kTileBound], grid))
+    hipLaunchKernelGGL(kb_pass_a_bound<T>, dim3(g), dim3(kBlock), 0, st, tab, ns);
+  TSPH_LAUNCH_CHECK("batch_density_alpha");
+  return 0;
+}
+
+template <class T>
+int batch_solver_begin(const TsphBatchInfo& bi, char* tab, int solve, hipStream_t st) {
+  hipLaunchKernelGGL(kb_solver_begin<T>, dim3(bi.nscenes), dim3(kBlock), 0, st, tab, bi.nscenes,
+                     solve);
+  TSPH_LAUNCH_CHECK("batch_solver_begin");
+  return 0;
+}
+
+template <class T>
+int batch_solver_half(const TsphBatchInfo& bi, char* tab, int kk, int half, int grid,
+                      hipStream_t st) {
+  const int g = grid_for(bi.tiles[kTileFluid], grid);
+  if (!g) return 0;
+  if (half & 1) hipLaunchKernelGGL(kb_solver_psi<T>, dim3(g), dim3(kBlock), 0, st, tab, bi.nscenes, kk);
+  if (half & 2)
+    hipLaunchKernelGGL(kb_solver_update<T>, dim3(g), dim3(kBlock), 0, st, tab, bi.nscenes, kk);
+  TSPH_LAUNCH_CHECK("batch_solver_iteration");
+  return 0;
+}
+
+template <class T>
+int batch_advect(const TsphBatchInfo& bi, char* tab, int grid, hipStream_t st) {
+  const int g = grid_for(bi.tiles[kTileFluid], grid);
+  if (!g) return 0;
+  hipLaunchKernelGGL(kb_advect<T>, dim3(g), dim3(kBlock), 0, st, tab, bi.nscenes);
+  hipLaunchKernelGGL(kb_surface_tension<T>, dim3(g), dim3(kBlock), 0, st, tab, bi.nscenes);
+  TSPH_LAUNCH_CHECK("batch_advect");
+  return 0;
+}
+
+template <class T>
+int batch_tail(const TsphBatchInfo& bi, char* tab, int grid, hipStream_t st) {
+  const int g = grid_for(bi.tiles[kTileFluid], grid);
+  if (g) hipLaunchKernelGGL(kb_tail<T>, dim3(g), dim3(kBlock), 0, st, tab, bi.nscenes);
+  hipLaunchKernelGGL(kb_finalize_energy<T>, dim3(bi.nscenes), dim3(kBlock), 0, st, tab, bi.nscenes);
+  TSPH_LAUNCH_CHECK("batch_tail");
+  return 0;
+}
+
+template <class T>
+int batch_stop_state(const TsphBatchInfo& bi, char* tab, int* out, hipStream_t st) {
+  hipLaunchKernelGGL(kb_stop_state<T>, dim3((bi.nscenes + 63) / 64), dim3(64), 0, st, tab,
+                     bi.nscenes, out);
+  TSPH_LAUNCH_CHECK("batch_stop_state");
+  return 0;
+}
+
+#define TSPH_BATCH_INST(T)                                                                  \
+  template int batch_density_alpha<T>(const TsphBatchInfo&, char*, int, hipStream_t);      \
+  template int batch_solver_begin<T>(const TsphBatchInfo&, char*, int, hipStream_t);       \
+  template int batch_solver_half<T>(const TsphBatchInfo&, char*, int, int, int, hipStream_t); \
+  template int batch_advect<T>(const TsphBatchInfo&, char*, int, hipStream_t);             \
+  template int batch_tail<T>(const TsphBatchInfo&, char*, int, hipStream_t);               \
+  template int batch_stop_state<T>(const TsphBatchInfo&, char*, int*, hipStream_t);
+TSPH_BATCH_INST(float)
+TSPH_BATCH_INST(double)
+#undef TSPH_BATCH_INST
 
 template int relist<float>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template int relist<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);

@@ -109,6 +109,27 @@ def scenes():
     return out
 
 
+def batched_dambreaks(n_scenes=64, seed=0, n_min=1e4, n_max=2e5):
+    """Independent 3D DFSPH dam breaks for a batch: fluid counts log-uniform
+    in [n_min, n_max], block aspect ratios (x/y, z/y) uniform in [0.5, 2],
+    the tank's interior twice the block per axis, viscosity uniform in
+    [0.005, 0.05], r = 0.005, dt = 1e-4 (cs = 100); numpy default_rng(seed).
+    Returns [(name, scenario, config)]."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_scenes):
+        n = float(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
+        ax, az = rng.uniform(0.5, 2.0, 2)
+        mu = float(rng.uniform(0.005, 0.05))
+        ny = max(2, int(round((n / (ax * az)) ** (1.0 / 3.0))))
+        nx, nz = max(2, int(round(ax * ny))), max(2, int(round(az * ny)))
+        scn = tank(0.01, [2 * nx, 2 * ny, 2 * nz], 2, [([0, 0, 0], [nx, ny, nz], [1])], [1000],
+                   [0, -9.81, 0], _r(mu))
+        out.append((f"batch{seed}_{i:02d}_{nx}x{ny}x{nz}", scn, solver_config("DFSPH", cs=100)))
+    return out
+
+
 def main():
     for name, (scn, cfg) in scenes().items():
         with open(os.path.join(HERE, name + ".json"), "w") as f:

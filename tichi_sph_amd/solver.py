@@ -65,6 +65,8 @@ class EventTimer:
 
     def per_name(self):
         """{name: [total ms, calls]} (synchronises on the recorded events)."""
+        if self.records:
+            self.records[-1][2].synchronize()
         out = {}
         for name, e0, e1 in self.records:
             d = out.setdefault(name, [0.0, 0])
