@@ -228,6 +228,26 @@ int tsph_fbm_round(const TsphParams* p, TsphObject* fluid, TsphControl* ctrl, vo
 int tsph_tail(const TsphParams* p, TsphObject* fluid, TsphControl* ctrl, int32_t fbm,
               void* workspace, size_t ws_bytes, void* stream);
 
+/*
+ * The WCSPH block of sph_step (main.py:236-240): WC_pressure_val (fluid),
+ * WC_pressure_acce (f,f)(f,b) and SPH_advection_update_vel_adv
+ * (sph.py:169-193, 247-250), after the density solve, over this step's
+ * neighbour lists.
+ */
+int tsph_wcsph(const TsphParams* p, TsphObject* fluid, TsphObject* bound, TsphControl* ctrl,
+               void* stream);
+
+/*
+ * map_velocity (sph.py:382-397) onto a Grid (sph_obj.py:328-344): node I of
+ * the row-major grid of grid_shape[dim] nodes (last axis fastest) sits at
+ * sim_space_lb + I * grid_dist; grid_vel (T[nodes][4]) receives the sum of
+ * X_j / sph_psi_j * vel_j * W(|x_I - x_j|) over obj's particles in the cells
+ * around the node's cell, in the reference's order.  obj's cell table must
+ * be this step's (tsph_sort_object), as the reference's Ngrid is.
+ */
+int tsph_map_velocity(const TsphParams* p, const TsphObject* obj, const int32_t* grid_shape,
+                      double grid_dist, void* grid_vel, void* stream);
+
 /* ----------------------------------------------------------------------
  * Reverse mode (adjoint) of the DFSPH step.  The reference has no
  * autodiff; these entries differentiate the fused step above (sph_step,

@@ -73,6 +73,8 @@ def lib():
         L.oracle_kernel.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
         L.oracle_sph_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(StepStats)]
         L.oracle_consts.argtypes = [ctypes.POINTER(RawInit), ctypes.POINTER(OracleConsts)]
+        L.oracle_map_velocity.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_double,
+                                          ctypes.c_void_p]
         L.oracle_kernel_fn.restype = ctypes.c_double
         L.oracle_kernel_fn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.oracle_lattice.restype = ctypes.c_long
@@ -203,6 +205,14 @@ class Oracle:
 
     def kernel(self, name, obj, nobj=-1):
         assert lib().oracle_kernel(self.h, name.encode(), obj, nobj) == 0, name
+
+    def map_velocity(self, obj, shape, dist):
+        """map_velocity (sph.py:382-397) of object `obj` over a Grid of
+        `shape` nodes spaced `dist` from sim_space_lb: [prod(shape), dim]."""
+        shape = np.ascontiguousarray(shape, dtype=np.int32)
+        out = np.zeros((int(np.prod(shape)), len(shape)), dtype=self.dtype)
+        lib().oracle_map_velocity(self.h, obj, shape.ctypes.data, float(dist), out.ctypes.data)
+        return out
 
     def step(self, flags=STEP_CFL):
         st = StepStats()

@@ -346,6 +346,49 @@ struct Sim {
     }
   }
 
+  // map_velocity, sph.py:382-397, over a Grid (sph_obj.py:328-344): node I
+  // sits at lb + I * dist (the field's float arithmetic); its velocity is the
+  // sum of X_j / psi_j * vel_j * W(|x_I - x_j|) over the candidates of object
+  // oi in the cells around the node's own cell, in template order.  Nodes
+  // are row-major (last axis fastest); out is [nodes][dim].
+  void map_velocity(int oi, const int* shape, T dist, T* out) {
+    const Obj<T>& no = obj[oi];
+    const int d = D();
+    long total = 1;
+    for (int a = 0; a < d; ++a) total *= shape[a];
+    par_for((int)total, [&](int idx) {
+      int I[3] = {0, 0, 0};
+      long r = idx;
+      for (int a = d - 1; a >= 0; --a) {
+        I[a] = (int)(r % shape[a]);
+        r /= shape[a];
+      }
+      T gp[3] = {T(0), T(0), T(0)};
+      for (int a = 0; a < d; ++a) gp[a] = T(c.lb[a]) + T(I[a]) * dist;
+      int node[3] = {0, 0, 0};
+      node_encode(gp, node);
+      T v[3] = {T(0), T(0), T(0)};
+      for (int t = 0; t < c.ntemp; ++t) {
+        int cell[3] = {0, 0, 0};
+        for (int a = 0; a < d; ++a) cell[a] = wrap_add(node[a], c.tmpl[t][a]);
+        int code = dim_encode(cell);
+        if (!(0 < code && code < c.node_num)) continue;
+        int cnt = grid.node_part_count[code];
+        for (int q = 0; q < cnt; ++q) {
+          int shift = grid.node_part_shift[code] + q;
+          if (grid.part_uid_in_node[shift] != no.uid) continue;
+          int j = grid.part_pid_in_node[shift];
+          T x[3] = {T(0), T(0), T(0)};
+          for (int a = 0; a < d; ++a) x[a] = gp[a] - no.pos[(size_t)j * d + a];
+          T w = km.W(norm_of(x));
+          T sc = X(no, j) / sph_psi(no, j);
+          for (int a = 0; a < d; ++a) v[a] = v[a] + (sc * no.vel[(size_t)j * d + a]) * w;
+        }
+      }
+      for (int a = 0; a < d; ++a) out[(size_t)idx * d + a] = v[a];
+    });
+  }
+
   template <class F>
   void par_for(int n, F&& f) {
 #ifdef _OPENMP
@@ -1180,6 +1223,15 @@ double oracle_scalar(void* p, int oi, int which, int set, double value) {
 int oracle_kernel(void* p, const char* name, int oi, int ni) {
   Handle* h = (Handle*)p;
   return h->precision == 8 ? S<double>(h)->kernel(name, oi, ni) : S<float>(h)->kernel(name, oi, ni);
+}
+
+int oracle_map_velocity(void* p, int oi, const int* shape, double dist, void* out) {
+  Handle* h = (Handle*)p;
+  if (h->precision == 8)
+    S<double>(h)->map_velocity(oi, shape, dist, (double*)out);
+  else
+    S<float>(h)->map_velocity(oi, shape, (float)dist, (float*)out);
+  return 0;
 }
 
 int oracle_sph_step(void* p, int flags, StepStats* st) {

@@ -103,7 +103,8 @@ EXPORTED = [
     "tsph_adj_solver_iteration", "tsph_adj_advect", "tsph_adj_density_alpha", "tsph_adj_end",
     "tsph_batch_table_bytes", "tsph_batch_prepare", "tsph_batch_sort", "tsph_batch_density_alpha",
     "tsph_batch_solver_begin", "tsph_batch_solver_psi", "tsph_batch_solver_update",
-    "tsph_batch_advect", "tsph_batch_tail", "tsph_batch_stop_state",
+    "tsph_batch_advect", "tsph_batch_tail", "tsph_batch_stop_state", "tsph_map_velocity",
+    "tsph_wcsph",
 ]
 
 _lib = None
@@ -167,6 +168,8 @@ def load(path=None):
         "tsph_batch_advect": [BI, _vp, _i32, _vp],
         "tsph_batch_tail": [BI, _vp, _i32, _vp],
         "tsph_batch_stop_state": [BI, _vp, _vp, _vp],
+        "tsph_map_velocity": [P, O, _vp, _f64, _vp, _vp],
+        "tsph_wcsph": [P, O, O, C, _vp],
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)

@@ -32,6 +32,10 @@ int solver_half(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, 
 template <class T>
 int relist(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template <class T>
+int map_velocity(const TsphParams&, const TsphObject*, const int*, double, void*, hipStream_t);
+template <class T>
+int wcsph(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, hipStream_t);
+template <class T>
 int batch_sort(const TsphBatchInfo&, char*, TsphObject*, int, int, hipStream_t);
 template <class T>
 int batch_density_alpha(const TsphBatchInfo&, char*, int, hipStream_t);
@@ -353,6 +357,25 @@ int tsph_adj_density_alpha(const TsphParams* p, TsphObject* f, TsphObject* b, Ts
   if (int rc = check_obj(b, "tsph_adj_density_alpha")) return rc;
   if (int rc = check_adj(p, adj, "tsph_adj_density_alpha")) return rc;
   return DISPATCH(p, adj_density_alpha, *p, f, b, adj, (hipStream_t)stream);
+}
+
+int tsph_wcsph(const TsphParams* p, TsphObject* f, TsphObject* b, TsphControl* ctrl,
+               void* stream) {
+  if (int rc = check_params(p)) return rc;
+  if (int rc = check_obj(f, "tsph_wcsph")) return rc;
+  if (int rc = check_obj(b, "tsph_wcsph")) return rc;
+  TSPH_REQUIRE(p->wc_gamma != 0, TSPH_EINVAL, "tsph_wcsph: solver_wc_gamma must be non-zero");
+  return DISPATCH(p, wcsph, *p, f, b, ctrl, (hipStream_t)stream);
+}
+
+int tsph_map_velocity(const TsphParams* p, const TsphObject* obj, const int32_t* grid_shape,
+                      double grid_dist, void* grid_vel, void* stream) {
+  if (int rc = check_params(p)) return rc;
+  if (int rc = check_obj(obj, "tsph_map_velocity")) return rc;
+  TSPH_REQUIRE(grid_shape && grid_vel, TSPH_EINVAL, "tsph_map_velocity: null grid");
+  for (int d = 0; d < p->dim; ++d)
+    TSPH_REQUIRE(grid_shape[d] >= 0, TSPH_EINVAL, "tsph_map_velocity: negative grid shape");
+  return DISPATCH(p, map_velocity, *p, obj, grid_shape, grid_dist, grid_vel, (hipStream_t)stream);
 }
 
 size_t tsph_batch_table_bytes(int32_t precision, int32_t nscenes) {

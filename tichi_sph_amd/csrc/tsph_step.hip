@@ -834,6 +834,152 @@ int tail(const TsphParams& prm, TsphObject* f, TsphControl* ctrl, int fbm, void*
 }
 
 // ---------------------------------------------------------------------------
+// WCSPH block (main.py:236-240, commented in the reference's sph_step):
+// WC_pressure_val (fluid), WC_pressure_acce (f,f)(f,b) and
+// SPH_advection_update_vel_adv.  Boundary pressure is never set by the
+// reference (WC_pressure_val runs for the fluid only), so it reads 0.
+// Strict arithmetic, the reference's order.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_wc_pressure(KP<T> k, int n, const T* __restrict__ dens, const T* __restrict__ rdens,
+                  T* __restrict__ pressure) {
+  TSPH_STRICT
+  int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  // (rho0 * cs^2 / gamma) * ((rho / rho0)^7 - 1), clamped at 0 (sph.py:169-175);
+  // the integer power by repeated squaring
+  T cs2 = (T)(k.cs * k.cs);
+  T ratio = dens[i] / rdens[i];
+  T res = T(1), b = ratio;
+  int e = 7;
+  while (e > 0) {
+    if (e & 1) res = res * b;
+    b = b * b;
+    e >>= 1;
+  }
+  T pr = (rdens[i] * cs2 / (T)k.wc_gamma) * (res - T(1));
+  if (pr < T(0)) pr = T(0);
+  pressure[i] = pr;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_wc_acce(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos, const T* __restrict__ fmass,
+              const T* __restrict__ fdens, const T* __restrict__ fpress, const int* __restrict__ fstart,
+              const V4<T>* __restrict__ bpos, const T* __restrict__ bmass,
+              const T* __restrict__ bdens, const T* __restrict__ bpress,
+              const int* __restrict__ bstart, const int* __restrict__ nbr_f,
+              const int* __restrict__ nbr_b, const int* __restrict__ nbr_nf,
+              const int* __restrict__ nbr_nb, const V4<T>* __restrict__ vel,
+              V4<T>* __restrict__ acce_adv, V4<T>* __restrict__ vel_adv,
+              const TsphControl* ctrl) {
+  TSPH_STRICT
+  int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const V4<T> p = fpos[i];
+  V4<T> a = acce_adv[i];
+  const T pi = fpress[i], di = fdens[i];
+  auto term = [&](const V4<T>& q, T mj, T dj, T pj) {
+    T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    T r = s_sqrt(s_r2(dx, dy, dz));
+    T pt = pi / (di * di) + pj / (dj * dj);
+    T wg = s_Wg(k, r);
+    a.x = a.x + -pt * mj * dx / r * wg;
+    a.y = a.y + -pt * mj * dy / r * wg;
+    if (k.dim == 3) a.z = a.z + -pt * mj * dz / r * wg;
+  };
+  for_fluid_nbrs(k, i, cap, nbr_nf[i], nbr_f, fstart, fpos, p,
+                 [&](int j) { term(fpos[j], fmass[j], fdens[j], fpress[j]); });
+  for_bound_nbrs(k, i, cap, nbr_nb[i], nbr_b, bstart, bpos, p,
+                 [&](int j) { term(bpos[j], bmass[j], bdens[j], bpress[j]); });
+  acce_adv[i] = a;
+  const T dt = (T)ctrl->dt;
+  const V4<T> v = vel[i];
+  vel_adv[i] = V4<T>{v.x + dt * a.x, v.y + dt * a.y, v.z + dt * a.z, T(0)};
+}
+
+template <class T>
+int wcsph(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl* ctrl,
+          hipStream_t st) {
+  KP<T> k = make_kp<T>(prm);
+  if (f->n == 0) return 0;
+  int nb = blocks_for(f->n);
+  hipLaunchKernelGGL(k_wc_pressure<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n,
+                     (const T*)f->sph_density, (const T*)f->rest_density, (T*)f->pressure);
+  hipLaunchKernelGGL(k_wc_acce<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
+                     (const V4<T>*)f->pos, (const T*)f->mass, (const T*)f->sph_density,
+                     (const T*)f->pressure, f->cell_start, (const V4<T>*)b->pos,
+                     (const T*)b->mass, (const T*)b->sph_density, (const T*)b->pressure,
+                     b->cell_start, f->nbr_f, f->nbr_b, f->nbr_nf, f->nbr_nb,
+                     (const V4<T>*)f->vel, (V4<T>*)f->acce_adv, (V4<T>*)f->vel_adv, ctrl);
+  TSPH_LAUNCH_CHECK("wcsph");
+  return 0;
+}
+template int wcsph<float>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, hipStream_t);
+template int wcsph<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// map_velocity (sph.py:382-397) over a Grid (sph_obj.py:328-344): one thread
+// per grid node (row-major, last axis fastest) at lb + I * dist; the node's
+// velocity sums X_j / psi_j * vel_j * W(|x_I - x_j|) over the object's
+// candidates in the cells around the node's cell, in template and in-cell
+// order (the object's cell table from this step's sort).  Strict arithmetic.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_map_velocity(KP<T> k, long total, int g0, int g1, int g2, T dist,
+                   const V4<T>* __restrict__ pos, const V4<T>* __restrict__ vel,
+                   const T* __restrict__ psi, const int* __restrict__ cstart,
+                   V4<T>* __restrict__ out) {
+  TSPH_STRICT
+  const long idx = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= total) return;
+  int I[3] = {0, 0, 0};
+  if (k.dim == 3) {
+    I[2] = (int)(idx % g2);
+    long r = idx / g2;
+    I[1] = (int)(r % g1);
+    I[0] = (int)(r / g1);
+  } else {
+    I[1] = (int)(idx % g1);
+    I[0] = (int)(idx / g1);
+  }
+  T gp[3] = {T(0), T(0), T(0)};
+  for (int d = 0; d < k.dim; ++d) gp[d] = k.lb[d] + T(I[d]) * dist;
+  int c[3];
+  node_encode(k, gp[0], gp[1], gp[2], c);
+  T vx = 0, vy = 0, vz = 0;
+  walk_cells(k, cstart, c, [&](int j) {
+    const V4<T> q = pos[j];
+    const T w = s_W(k, s_sqrt(s_r2(gp[0] - q.x, gp[1] - q.y, gp[2] - q.z)));
+    const T sc = q.w / psi[j];
+    const V4<T> v = vel[j];
+    vx = vx + (sc * v.x) * w;
+    vy = vy + (sc * v.y) * w;
+    vz = vz + (sc * v.z) * w;
+  });
+  out[idx] = V4<T>{vx, vy, vz, T(0)};
+}
+
+template <class T>
+int map_velocity(const TsphParams& prm, const TsphObject* o, const int* shape, double dist,
+                 void* out, hipStream_t st) {
+  KP<T> k = make_kp<T>(prm);
+  long total = 1;
+  for (int d = 0; d < prm.dim; ++d) total *= shape[d];
+  if (total <= 0) return 0;
+  const T* psi = (const T*)(prm.solver_type == TSPH_VFSPH ? o->sph_compression : o->sph_density);
+  hipLaunchKernelGGL(k_map_velocity<T>, dim3((unsigned)((total + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                     st, k, total, shape[0], shape[1], prm.dim == 3 ? shape[2] : 1, (T)dist,
+                     (const V4<T>*)o->pos, (const V4<T>*)o->vel, psi, o->cell_start, (V4<T>*)out);
+  TSPH_LAUNCH_CHECK("map_velocity");
+  return 0;
+}
+template int map_velocity<float>(const TsphParams&, const TsphObject*, const int*, double, void*, hipStream_t);
+template int map_velocity<double>(const TsphParams&, const TsphObject*, const int*, double, void*, hipStream_t);
+
+// ---------------------------------------------------------------------------
 // Grouped (batched) launches: the kernels above over every scene of a table
 // (tsph_host.h:SceneEntry), with the single-scene launchers' arguments.
 // ---------------------------------------------------------------------------

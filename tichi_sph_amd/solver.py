@@ -13,6 +13,7 @@ drop-in use and as a parity anchor.
 """
 import ctypes
 import json
+import warnings
 
 import numpy as np
 import torch
@@ -75,9 +76,46 @@ class EventTimer:
         return out
 
 
+class GlobalVariable:
+    """The run-time counters of sph_obj.py:347-369 that the step loop and the
+    writers use.  step_counter counts sph_steps (the reference declares it
+    but never advances it)."""
+
+    def __init__(self):
+        self.time_count = 0.0
+        self.time_counter = 0
+        self.step_counter = 0
+        self.frame_div_iter = 0
+        self.frame_incom_iter = 0
+        self.div_iter_count = 0
+        self.incom_iter_count = 0
+
+
+def inlet_helper_get_dt(norm, speed, relaxing_factor, config):
+    """main.py:24-39: the time for an inlet's sheet to advance one particle
+    spacing, (sqrt(v^2 + 2 a r) - v) / a with a = the gravity component
+    along the inlet's normal (r / v without acceleration), times the
+    relaxing factor."""
+    norm = np.array(norm) / np.linalg.norm(np.array(norm))
+    if speed < 0:
+        norm, speed = -norm, -speed
+    acc_vec = np.array(config.gravity[: config.dim], dtype=np.float32)
+    acc = np.dot(norm, acc_vec)
+    r = config.part_size[1]
+    if acc < 1e-6:
+        if speed > 0:
+            return r * relaxing_factor / speed
+        return float("inf")
+    return ((speed ** 2 + 2 * acc * r) ** 0.5 - speed) / acc * relaxing_factor
+
+
 class Simulation:
     """Owns the two particle objects (fluid uid 0, boundary uid 1), the
-    control block, the workspace and the reference-layout Ngrid."""
+    control block, the workspace and the reference-layout Ngrid.
+
+    device="cpu" gives a host-only Simulation for building and inspecting
+    scenes; every compute call on it raises (the library computes on the
+    GPU only)."""
 
     def __init__(self, config, device="cuda", precision=4, fbm=False, wcsph=False, cfl=True,
                  list_cap_f=64, list_cap_b=48, fluid_max=None, bound_max=None):
@@ -86,6 +124,7 @@ class Simulation:
         self.precision = precision
         self.lib = _lib.load()
         self.fbm = fbm
+        self.wcsph = wcsph
         self.cfl = cfl
         self.params = config.params(precision, list_cap_f, list_cap_b)
         fm = config.fluid_max_part_num if fluid_max is None else fluid_max
@@ -96,24 +135,34 @@ class Simulation:
         self.ws = torch.zeros(int(self.lib.tsph_workspace_bytes(ctypes.byref(self.params), cap)),
                               dtype=torch.uint8, device=self.device)
         self.ctrl = torch.zeros(int(self.lib.tsph_control_bytes()), dtype=torch.uint8, device=self.device)
-        self._ctrl_host = torch.zeros(self.ctrl.numel(), dtype=torch.uint8).pin_memory()
-        self._stop_host = torch.zeros(16, dtype=torch.uint8).pin_memory()
+        pin = self.device.type == "cuda"
+        self._ctrl_host = torch.zeros(self.ctrl.numel(), dtype=torch.uint8, pin_memory=pin)
+        self._stop_host = torch.zeros(16, dtype=torch.uint8, pin_memory=pin)
         self.ngrid = Ngrid(self)
         self.bound_dirty = True
         self.pred = [max(config.iter_threshold_min, 1), max(config.iter_threshold_min, 1)]
         if config.iter_threshold_max + 1 > _lib.TSPH_MAX_ITER_LOG:
             raise Exception(f"solver_iter_threshold_max must be < {_lib.TSPH_MAX_ITER_LOG}")
         self.set_dt(config.dt)
-        self.time_count = 0.0
+        self.globalvar = GlobalVariable()
+        self.scenario_wait_buffer = []  # objects with a positive `time`, added later
+        self.inlets = []                # inlets pouring particles in (main.py:14)
         # optional per-call HIP-event timing (an EventTimer)
         self.timer = None
         self.calls = 0  # library calls issued through _call (a change counter)
 
     # ----------------------------------------------------------------- helpers
+    def _require_gpu(self, what):
+        if self.device.type != "cuda":
+            raise _lib.TsphError(f"{what}: this Simulation is host-only (device={self.device}); "
+                                 "the library computes on the GPU")
+
     def stream_ptr(self):
+        self._require_gpu("compute")
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
     def _call(self, name, fn, *args):
+        self._require_gpu(name)
         self.calls += 1
         if self.timer is None:
             _lib.check(fn(*args), name)
@@ -135,6 +184,10 @@ class Simulation:
         self._ctrl_host.copy_(self.ctrl, non_blocking=True)
         torch.cuda.current_stream(self.device).synchronize()
         return _lib.TsphControl.from_buffer_copy(self._ctrl_host.numpy().tobytes())
+
+    @property
+    def time_count(self):
+        return self.globalvar.time_count
 
     def set_dt(self, dt):
         raw = self.control() if self.ctrl.is_cuda else _lib.TsphControl()
@@ -158,33 +211,116 @@ class Simulation:
         return int(a[0]), int(a[1]), int(a[3])  # it, stop, general_flag
 
     # --------------------------------------------------------------- scenario
+    def get_obj_from_str(self, s):
+        """main.py:16-22."""
+        if s == "fluid":
+            return self.fluid
+        if s == "bound":
+            return self.bound
+        print("scenario WARNING: object does not exist (name:", s, ")")
+        return None
+
     def add_to_scenario(self, obj_str, param):
-        """main.py:42-60 for the `cube` and `box` object types."""
-        obj = self.fluid if obj_str == "fluid" else self.bound if obj_str == "bound" else None
+        """main.py:42-60: `cube`, `box`, `ply` (vertices of a PLY file) and
+        `inlet` (registered; refresh_inlets emits its sheets)."""
+        from .ply import read_ply
+
+        obj = self.get_obj_from_str(obj_str)
         if obj is None:
             print("scenario WARNING: obj '", obj_str, "' non-exist.")
             return
-        color = int(param.get("color", "0x0"), 16)
-        if param["type"] == "cube":
+        kind = param["type"]
+        if kind == "cube":
             obj.scene_add_cube(param["start_pos"], param["end_pos"], param["volume_frac"], param["vel"],
-                               color, param["particle_relaxing_factor"])
-        elif param["type"] == "box":
+                               int(param["color"], 16), param["particle_relaxing_factor"])
+        elif kind == "box":
             obj.scene_add_box(param["start_pos"], param["end_pos"], param["layers"], param["volume_frac"],
-                              param["vel"], color, param["particle_relaxing_factor"])
+                              param["vel"], int(param["color"], 16), param["particle_relaxing_factor"])
+        elif kind == "ply":
+            verts = read_ply(param["file_name"])
+            obj.push_part_from_ply(len(verts), verts, param["volume_frac"], param["vel"], int(param["color"], 16))
+        elif kind == "inlet":
+            if np.linalg.norm(np.array(param["norm"])) < 1e-6:
+                raise Exception("inlet ERROR: magnitute of attribute 'norm' should not be zero.")
+            param["obj"] = obj_str
+            param["dt"] = inlet_helper_get_dt(param["norm"], param["speed"], param["particle_relaxing_factor"],
+                                              self.config)
+            param["t_pre"] = self.globalvar.time_count - param["dt"]
+            self.inlets.append(param)
         else:
-            raise Exception("scenario ERROR: object type '%s' is not supported by this build" % param["type"])
-        if obj is self.bound:
+            raise Exception("scenario ERROR: object type '%s' is not supported" % kind)
+        if obj is self.bound and kind != "inlet":
             self.bound_dirty = True
 
     def init_scenario(self):
-        """main.py:62-79: objects without a positive `time` are added now."""
+        """main.py:62-79: objects without a positive `time` are added now,
+        the others wait in scenario_wait_buffer."""
         sb = self.config.scenario_buffer
-        for part in sb:
-            if part == "sim_env":
-                continue
-            for param in sb[part]["objs"]:
-                if "time" not in param or param["time"] <= 0.0:
-                    self.add_to_scenario(part, param)
+        try:
+            for part in sb:
+                if part == "sim_env":
+                    continue
+                for param in sb[part]["objs"]:
+                    if "time" not in param or param["time"] <= 0.0:
+                        self.add_to_scenario(part, param)
+                    else:
+                        param["_part"] = part
+                        self.scenario_wait_buffer.append(param)
+        except Exception as e:
+            raise Exception("scenario ERROR: no scenario file or scenario file invalid.") from e
+
+    def refresh_inlets(self):
+        """main.py:82-101: every inlet whose interval has passed emits one
+        sheet; inlets past their end_time are removed."""
+        for i in range(len(self.inlets) - 1, -1, -1):
+            inlet = self.inlets[i]
+            t = self.globalvar.time_count
+            if "end_time" in inlet and t >= inlet["end_time"]:
+                self.inlets.pop(i)
+            elif t >= inlet["t_pre"] + inlet["dt"]:
+                try:
+                    inlet["t_pre"] = t
+                    self.get_obj_from_str(inlet["obj"]).scene_add_from_inlet(
+                        inlet["center"], inlet["size"], inlet["norm"], inlet["speed"], inlet["volume_frac"],
+                        int(inlet["color"], 16), inlet["particle_relaxing_factor"])
+                    if inlet["obj"] == "bound":
+                        self.bound_dirty = True
+                except Exception:
+                    warnings.warn("inlet WARNING: refresh_inlets failed for:" + json.dumps(inlet), RuntimeWarning)
+                    self.inlets.pop(i)
+
+    def refresh_scenario(self):
+        """main.py:103-112: add waiting objects whose time has come, then
+        refresh the inlets."""
+        for i in range(len(self.scenario_wait_buffer) - 1, -1, -1):
+            param = self.scenario_wait_buffer[i]
+            try:
+                if param["time"] <= self.globalvar.time_count:
+                    self.add_to_scenario(param["_part"], param)
+                    self.scenario_wait_buffer.pop(i)
+            except Exception:
+                warnings.warn("scenario WARNING: invalid scenario object:" + json.dumps(param), RuntimeWarning)
+                self.scenario_wait_buffer.pop(i)
+        self.refresh_inlets()
+
+    def run_step(self):
+        """One GUI frame of main.py:339-356: steps (refresh_scenario,
+        cfl_condition, sph_step) until the simulated time reaches the next
+        frame at gui_fps.  Returns the number of steps taken."""
+        g = self.globalvar
+        g.time_counter += 1
+        steps = 0
+        while g.time_count < g.time_counter / self.config.gui_fps:
+            self.refresh_scenario()
+            st = self.sph_step(stats=True)
+            g.time_count += st.dt
+            g.step_counter += 1
+            g.div_iter_count, g.incom_iter_count = st.div_iter, st.incom_iter
+            g.frame_div_iter += st.div_iter
+            g.frame_incom_iter += st.incom_iter
+            self.last_stats = st
+            steps += 1
+        return steps
 
     # ------------------------------------------------------------ fused step
     def _sort_bound(self):
@@ -264,6 +400,8 @@ class Simulation:
         div_it = self._solve(0)
         self._call("advect", self.lib.tsph_advect, P, f, b, self.ctrl_ptr(), ws, wsb, st)
         inc_it = self._solve(1)
+        if self.wcsph:  # main.py:236-240
+            self._call("wcsph", self.lib.tsph_wcsph, P, f, b, self.ctrl_ptr(), st)
         if self.fbm:
             rounds = 0
             while rounds < 1000:
@@ -278,13 +416,15 @@ class Simulation:
         return div_it, inc_it
 
     # -------------------------------------------------- reference-order step
-    def sph_step_reference(self, wcsph=False, fbm=None):
+    def sph_step_reference(self, wcsph=None, fbm=None):
         """main.py:175-256 kernel by kernel through the sph.py surface
         (cfl_condition first, as run_step does at main.py:348)."""
         from . import sph as K
 
         fbm = self.fbm if fbm is None else fbm
+        wcsph = self.wcsph if wcsph is None else wcsph
         fluid, bound, ngrid = self.fluid, self.bound, self.ngrid
+        fluid.cell_table_ok = False  # this path walks the Ngrid, not the cell order
         cfg = self.config
         if self.cfl:
             K.cfl_condition(self, fluid)

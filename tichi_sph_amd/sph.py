@@ -54,3 +54,18 @@ for _n in _PAIR:
     globals()[_n] = _make_pair_kernel(_n)
 
 __all__ = _OBJ + _PAIR
+
+
+def map_velocity(sim, grid, nobj):
+    """map_velocity(ngrid, grid, nobj), sph.py:382-397: SPH interpolation of
+    nobj's velocity onto the grid's nodes (grid.vel).  nobj's cell table must
+    be this step's (after Simulation.sph_step), like the reference's Ngrid."""
+    if not getattr(nobj, "cell_table_ok", False):
+        raise _lib.TsphError("map_velocity: the object's cell table is stale (run Simulation.sph_step first)")
+    shape = (ctypes.c_int32 * 3)(*(list(grid.shape) + [1] * (3 - len(grid.shape))))
+    sim._call("map_velocity", sim.lib.tsph_map_velocity, ctypes.byref(sim.params), ctypes.byref(nobj.struct()),
+              shape, float(grid.dist), ctypes.c_void_p(grid.vel.data_ptr()), sim.stream_ptr())
+
+
+__all__.append("map_velocity")
+

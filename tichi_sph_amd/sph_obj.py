@@ -16,6 +16,12 @@ import torch
 
 from . import _lib
 
+
+def _np_normalize(v):
+    """np_normalize, sph_util.py:74-75."""
+    return v / np.linalg.norm(v)
+
+
 VEC_FIELDS = ("pos", "vel", "vel_adv", "acce_adv", "acce", "normal", "alpha_1")
 SCALAR_FIELDS = ("mass", "rest_density", "rest_volume", "W", "sph_compression", "sph_density",
                  "psi_adv", "alpha", "alpha_2", "kappa", "vol", "fbm_zeta", "pressure")
@@ -68,6 +74,7 @@ class Fluid:
         self.t["pid_b"] = zi(cap)
         self._struct = _lib.TsphObject()
         self._refresh_struct()
+        self.cell_table_ok = False  # cell_start matches the storage (set by the sort)
 
     # ------------------------------------------------------------------ layout
     def _refresh_struct(self):
@@ -83,8 +90,10 @@ class Fluid:
         return self._struct
 
     def adopt_swaps(self):
-        """After the library swapped front/back pointers inside the descriptor,
-        swap the tensors so that self.t matches the descriptor again."""
+        """After the library sorted the object and swapped front/back pointers
+        inside the descriptor, swap the tensors so that self.t matches the
+        descriptor again."""
+        self.cell_table_ok = True
         s = self._struct
         for name in ("pos", "vel", "mass", "rest_density", "rest_volume", "volume_frac", "pid"):
             if getattr(s, name) != self.t[name].data_ptr():
@@ -169,6 +178,33 @@ class Fluid:
         self.push_matrix(box, np.asarray(start_pos, dtype=np.float64) + padding, spacing,
                          volume_frac, vel, color)
 
+    def scene_add_from_inlet(self, center, size, norm, speed, volume_frac, color, relaxing_factor):
+        """sph_obj.py:117-137: one sheet of particles on the inlet's plane.
+        u lies in the x-z plane orthogonal to `norm`, v = normalize(u x norm);
+        the sheet is size[0] along u by size[1] along v, centred on `center`,
+        moving with velocity norm * speed.  3D only."""
+        spacing = self.config.part_size[1] * relaxing_factor
+        matrix_shape, padding = self.scene_add_help_centering([0] * len(size), size, spacing)
+        if len(matrix_shape) != 2:
+            raise Exception("scenario ERROR: can only add 3D inlets.")
+        norm = list(norm)
+        if norm[0] == 0:
+            u = np.array([1, 0, 0])
+        else:
+            u = _np_normalize(np.array([-norm[2] / norm[0], 0, 1]))
+        v = _np_normalize(np.cross(u, norm))
+        size = np.array(size) - padding
+        start = np.array(center) - (size[0] * u + size[1] * v) / 2
+        seq = [start + (float(i) * u + float(j) * v) * spacing
+               for i in range(matrix_shape[0]) for j in range(matrix_shape[1])]
+        pos_seq = np.array(seq).reshape(-1, 3)
+        vel = np.array(norm) * speed
+        self.push_part_seq(len(pos_seq), pos_seq, volume_frac, vel, color)
+
+    def push_part_from_ply(self, p_sum, pos_seq, volume_frac, vel, color):
+        """sph_obj.py:153-154."""
+        self.push_part_seq(p_sum, pos_seq, volume_frac, vel, color)
+
     def push_matrix(self, matrix, start_position, spacing, volume_frac, vel, color):
         """sph_obj.py:158-163: np.where order (last axis fastest), f64 positions."""
         if len(matrix.shape) != self.config.dim:
@@ -203,6 +239,7 @@ class Fluid:
         self.t["pid"][sl] = torch.arange(n0, n1, dtype=torch.int32, device=self.device)
         self.color += [color] * int(pushed_part_num)
         self.part_num = n1
+        self.cell_table_ok = False
         self.update_mass_all()
 
     def update_mass_all(self):
