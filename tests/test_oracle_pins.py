@@ -373,6 +373,62 @@ def test_two_particle_closed_forms():
     assert o.get(0, "vel_adv")[1] == pytest.approx(-dvi, rel=1e-10)
 
 
+def test_wcsph_tait_equation_closed_form():
+    """WC_pressure_val (sph.py:169-175) is the Tait equation of state
+    p = rho0 c^2 / gamma ((rho / rho0)^7 - 1), clamped at 0 (Becker and
+    Teschner 2007), with c = solver_cs and gamma = solver_wc_gamma."""
+    cb, sb = load_scene("dambreak_2d_8k")
+    o = O.Oracle(cb, sb, 8)
+    o.init_scenario()
+    n = o.n[0]
+    rho = np.linspace(950.0, 1050.0, n)
+    o.set(0, "sph_density", rho)
+    o.kernel("WC_pressure_val", 0)
+    c, g = float(cb["solver_cs"]), float(cb["solver_wc_gamma"])
+    expect = np.maximum(1000.0 * c * c / g * ((rho / 1000.0) ** 7 - 1.0), 0.0)
+    p = o.get(0, "pressure")
+    assert np.all(p[rho < 1000.0] == 0) and p[-1] > 0
+    assert np.allclose(p, expect, rtol=1e-12, atol=1e-9)
+
+
+def test_wcsph_pressure_force_momentum_and_two_particle_form():
+    """WC_pressure_acce (sph.py:178-193): the (f,f) pressure forces are
+    pairwise antisymmetric (total momentum change exactly zero), and for two
+    particles a_1 = -(p1/rho1^2 + p2/rho2^2) m2 W'(r) x12 / r."""
+    cb, sb = load_scene("dambreak_3d_8k")
+    o = O.Oracle(cb, sb, 8)
+    o.init_scenario()
+    for k, args in (("clear_node", (0,)), ("encode", (0,)), ("encode", (1,)), ("mem_shift", (0,)),
+                    ("fill_node", (0,)), ("fill_node", (1,)), ("SPH_clean_value", (0,)),
+                    ("SPH_prepare_attr", (0, 0)), ("SPH_prepare_attr", (0, 1))):
+        o.kernel(k, *args)
+    n = o.n[0]
+    rng = np.random.default_rng(2)
+    o.set(0, "pressure", rng.uniform(0, 50, n))
+    o.kernel("WC_pressure_acce", 0, 0)
+    m = o.get(0, "mass")
+    a = o.get(0, "acce_adv")
+    mom = (m[:, None] * a).sum(0)
+    assert np.abs(a).max() > 0
+    assert np.all(np.abs(mom) <= 1e-12 * (m[:, None] * np.abs(a)).sum(0).max())
+    # two particles
+    dx = np.array([0.008, 0.003, 0.0])
+    o2 = _pair_oracle(dx)
+    for k, args in (("clear_node", (0,)), ("encode", (0,)), ("encode", (1,)), ("mem_shift", (0,)),
+                    ("fill_node", (0,)), ("fill_node", (1,)), ("SPH_clean_value", (0,)),
+                    ("SPH_prepare_attr", (0, 0))):
+        o2.kernel(k, *args)
+    o2.set(0, "pressure", np.array([30.0, 10.0]))
+    o2.kernel("WC_pressure_acce", 0, 0)
+    rho = o2.get(0, "sph_density")
+    r = float(np.linalg.norm(dx))
+    h = o2.c.kernel_h[1]
+    wg = O.lib().oracle_kernel_fn(1, 8, h, o2.c.kernel_sig[3], r)
+    x12 = -dx  # x_1 - x_2
+    a1 = -(30.0 / rho[0] ** 2 + 10.0 / rho[1] ** 2) * 1e-3 * wg * x12 / r
+    assert np.allclose(o2.get(0, "acce_adv")[0], a1, rtol=1e-12)
+
+
 def test_solver_fixed_point_and_stop_rule():
     # uncompressed lattice at rest, no gravity: psi clamps to 0, velocities stay 0,
     # the solves stop after exactly iter_min iterations.

@@ -405,8 +405,8 @@ int ngrid_build(const TsphParams& prm, TsphObject* const* objs, int nobj, int* n
 // Grouped (batched) sort: the kernels above over every scene of a table.
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ void __launch_bounds__(kBlock) kb_fill_counts(char* tab, int ns) {
-  for_tiles<T, kTileCell>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
+__global__ void __launch_bounds__(kBlock) kb_fill_counts(const char* __restrict__ tab, int ns) {
+  for_tiles<T, kTileCell>(tab, ns, [&](const SceneEntry<T>& e, int, Blk blk) {
     const long n = sort_len(e.k.node_num) + 1;  // cell_count[node_num + 2]
     const long base = (long)blk.b * kScanTile;
     for (int q = 0; q < 16; ++q) {
@@ -417,48 +417,48 @@ __global__ void __launch_bounds__(kBlock) kb_fill_counts(char* tab, int ns) {
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) kb_keys(char* tab, int ns, int cfl) {
-  for_tiles<T, kTileFluid>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
+__global__ void __launch_bounds__(kBlock) kb_keys(const char* __restrict__ tab, int ns, int cfl) {
+  for_tiles<T, kTileFluid>(tab, ns, [&](const SceneEntry<T>& e, int, Blk blk) {
     k_keys_body<T>(blk, e.k, (const V4<T>*)e.f.pos, (const V4<T>*)e.f.vel, e.f.n, e.L.key, e.L.rank,
                    e.f.cell_count, cfl, e.L.partial, e.ctrl);
   });
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) kb_finalize_cfl(char* tab, int ns) {
+__global__ void __launch_bounds__(kBlock) kb_finalize_cfl(const char* __restrict__ tab, int ns) {
   const int* firstf = (const int*)(tab + batch_entries_bytes<T>(ns)) + kTileFluid * (ns + 1);
-  for_tiles<T, kTileScene>(tab, ns, [&](SceneEntry<T>& e, int s, Blk blk) {
+  for_tiles<T, kTileScene>(tab, ns, [&](const SceneEntry<T>& e, int s, Blk blk) {
     k_finalize_body<T>(blk, e.k, (int)kFinCfl, e.L.partial, firstf[s + 1] - firstf[s], e.f.n,
                        e.ctrl);
   });
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) kb_scan_reduce(char* tab, int ns) {
-  for_tiles<T, kTileCell>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
+__global__ void __launch_bounds__(kBlock) kb_scan_reduce(const char* __restrict__ tab, int ns) {
+  for_tiles<T, kTileCell>(tab, ns, [&](const SceneEntry<T>& e, int, Blk blk) {
     k_scan_reduce_body(blk, e.f.cell_count, sort_len(e.k.node_num), e.L.scan_sums);
   });
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) kb_scan_apply(char* tab, int ns) {
-  for_tiles<T, kTileCell>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
+__global__ void __launch_bounds__(kBlock) kb_scan_apply(const char* __restrict__ tab, int ns) {
+  for_tiles<T, kTileCell>(tab, ns, [&](const SceneEntry<T>& e, int, Blk blk) {
     k_scan_apply_body(blk, e.f.cell_count, sort_len(e.k.node_num), e.L.scan_sums,
                       e.f.cell_start);
   });
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) kb_scatter(char* tab, int ns) {
-  for_tiles<T, kTileFluid>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
+__global__ void __launch_bounds__(kBlock) kb_scatter(const char* __restrict__ tab, int ns) {
+  for_tiles<T, kTileFluid>(tab, ns, [&](const SceneEntry<T>& e, int, Blk blk) {
     k_scatter_body(blk, e.L.key, e.L.rank, e.f.cell_start, e.f.pid, e.f.n, e.L.tmp_idx,
                    e.L.tmp_pid, e.L.tmp_key);
   });
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) kb_permute(char* tab, int ns) {
-  for_tiles<T, kTileFluid>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
+__global__ void __launch_bounds__(kBlock) kb_permute(const char* __restrict__ tab, int ns) {
+  for_tiles<T, kTileFluid>(tab, ns, [&](const SceneEntry<T>& e, int, Blk blk) {
     const TsphObject& o = e.f;
     k_permute_body<T>(blk, e.k.P, o.n, o.cell_start, e.L.tmp_idx, e.L.tmp_pid, e.L.tmp_key,
                       (const V4<T>*)o.pos, (const V4<T>*)o.vel, (const T*)o.mass,

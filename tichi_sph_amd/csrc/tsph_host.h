@@ -195,12 +195,16 @@ __device__ __forceinline__ int scene_of(const int* first, int ns, int t) {
 
 // Grid-stride loop over the tiles of one kind; f(entry, scene, blk).
 template <class T, int KIND, class F>
-__device__ __forceinline__ void for_tiles(char* tab, int ns, F&& f) {
-  SceneEntry<T>* E = (SceneEntry<T>*)tab;
+__device__ __forceinline__ void for_tiles(const char* __restrict__ tab, int ns, F&& f) {
+  // read-only here (the sort's pointer swap is a separate kernel), so the
+  // scene entries can be scalar loads
+  const SceneEntry<T>* __restrict__ E = (const SceneEntry<T>*)tab;
   const int* first = (const int*)(tab + batch_entries_bytes<T>(ns)) + KIND * (ns + 1);
   const int total = first[ns];
   for (int t = blockIdx.x; t < total; t += gridDim.x) {
-    const int s = scene_of(first, ns, t);
+    // uniform per block: readfirstlane lets the scene's constants live in
+    // scalar registers
+    const int s = __builtin_amdgcn_readfirstlane(scene_of(first, ns, t));
     const int f0 = first[s];
     f(E[s], s, Blk{t - f0, first[s + 1] - f0});
   }

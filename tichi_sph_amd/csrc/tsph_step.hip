@@ -20,23 +20,26 @@ struct PassAAcc {
   T w = 0, cp = 0, dn = 0, ax = 0, ay = 0, az = 0, a2 = 0;
   __device__ __forceinline__ bool add(const KP<T>& k, const V4<T>& p, const V4<T>& q, T mj, T vj,
                                       bool fluid) {
-    TSPH_STRICT
     T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-    T r = s_sqrt(s_r2(dx, dy, dz));
-    T Wr = s_W(k, r);
-    w = w + Wr;
-    cp = cp + Wr * vj;
-    dn = dn + Wr * mj;
-    if (!(r > T(0))) return false;
-    T wg = s_Wg(k, r);
-    T Xj = q.w;
-    ax = ax + Xj * dx / r * wg;
-    ay = ay + Xj * dy / r * wg;
-    az = az + Xj * dz / r * wg;
-    if (fluid) {
-      T wg2 = wg * wg;
-      a2 = a2 + wg2 * (Xj * Xj) / mj;
+    T r2;
+    {
+      TSPH_STRICT
+      r2 = s_r2(dx, dy, dz);
+      T r = s_sqrt(r2);
+      T Wr = s_W(k, r);
+      w = w + Wr;
+      cp = cp + Wr * vj;
+      dn = dn + Wr * mj;
+      if (!(r > T(0))) return false;
     }
+    T rr;
+    T g = f_grad(k, r2, &rr);
+    T Xj = q.w;
+    T xg = Xj * g;
+    ax += xg * dx;
+    ay += xg * dy;
+    az += xg * dz;
+    if (fluid) a2 += (g * g) * r2 * (Xj * Xj) * f_rcp(mj);
     return true;
   }
 };
@@ -989,14 +992,14 @@ __device__ __forceinline__ const T* psi_of(const SceneEntry<T>& e) {
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) kb_step_reset(char* tab, int ns) {
-  for_tiles<T, kTileScene>(tab, ns, [&](SceneEntry<T>& e, int, Blk) { step_reset_body(e.ctrl); });
+__global__ void __launch_bounds__(kBlock) kb_step_reset(const char* __restrict__ tab, int ns) {
+  for_tiles<T, kTileScene>(tab, ns, [&](const SceneEntry<T>& e, int, Blk) { step_reset_body(e.ctrl); });
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) kb_pass_a_fluid(char* tab, int ns) {
-  for_tiles<T, kTileFluid>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
-    TsphObject& f = e.f;
+__global__ void __launch_bounds__(kBlock) kb_pass_a_fluid(const char* __restrict__ tab, int ns) {
+  for_tiles<T, kTileFluid>(tab, ns, [&](const SceneEntry<T>& e, int, Blk blk) {
+    const TsphObject& f = e.f;
     const TsphObject& b = e.b;
     k_pass_a_fluid_body<T>(blk, e.k, f.n, f.capacity, (const V4<T>*)f.pos, (const T*)f.mass,
                            (const T*)f.rest_volume, f.cell_start, b.n, (const V4<T>*)b.pos,
@@ -1009,10 +1012,10 @@ __global__ void __launch_bounds__(kBlock) kb_pass_a_fluid(char* tab, int ns) {
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) kb_pass_a_bound(char* tab, int ns) {
-  for_tiles<T, kTileBound>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
+__global__ void __launch_bounds__(kBlock) kb_pass_a_bound(const char* __restrict__ tab, int ns) {
+  for_tiles<T, kTileBound>(tab, ns, [&](const SceneEntry<T>& e, int, Blk blk) {
     const TsphObject& f = e.f;
-    TsphObject& b = e.b;
+    const TsphObject& b = e.b;
     k_pass_a_bound_body<T>(blk, e.k, b.n, (const V4<T>*)b.pos, (const T*)b.mass,
                            (const T*)b.rest_volume, b.cell_start, (const V4<T>*)f.pos,
                            (const T*)f.mass, (const T*)f.rest_volume, f.cell_start, (T*)b.W,
@@ -1022,15 +1025,15 @@ __global__ void __launch_bounds__(kBlock) kb_pass_a_bound(char* tab, int ns) {
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) kb_solver_begin(char* tab, int ns, int solve) {
+__global__ void __launch_bounds__(kBlock) kb_solver_begin(const char* __restrict__ tab, int ns, int solve) {
   for_tiles<T, kTileScene>(tab, ns,
-                           [&](SceneEntry<T>& e, int, Blk) { solver_begin_body(e.k, e.ctrl, solve); });
+                           [&](const SceneEntry<T>& e, int, Blk) { solver_begin_body(e.k, e.ctrl, solve); });
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) kb_solver_psi(char* tab, int ns, int kk) {
-  for_tiles<T, kTileFluid>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
-    TsphObject& f = e.f;
+__global__ void __launch_bounds__(kBlock) kb_solver_psi(const char* __restrict__ tab, int ns, int kk) {
+  for_tiles<T, kTileFluid>(tab, ns, [&](const SceneEntry<T>& e, int, Blk blk) {
+    const TsphObject& f = e.f;
     k_solver_psi_body<T>(blk, e.k, f.n, f.capacity, kk, (const V4<T>*)f.pos,
                          (const V4<T>*)f.vel_adv, psi_of(e), (const T*)f.rest_density,
                          (const T*)f.alpha, f.cell_start, (const V4<T>*)e.b.pos, e.b.cell_start,
@@ -1040,9 +1043,9 @@ __global__ void __launch_bounds__(kBlock) kb_solver_psi(char* tab, int ns, int k
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) kb_solver_update(char* tab, int ns, int kk) {
-  for_tiles<T, kTileFluid>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
-    TsphObject& f = e.f;
+__global__ void __launch_bounds__(kBlock) kb_solver_update(const char* __restrict__ tab, int ns, int kk) {
+  for_tiles<T, kTileFluid>(tab, ns, [&](const SceneEntry<T>& e, int, Blk blk) {
+    const TsphObject& f = e.f;
     k_solver_update_body<T>(blk, e.k, f.n, f.capacity, kk, (const V4<T>*)f.pos,
                             (const T*)f.mass, (const T*)f.kappa, f.cell_start,
                             (const V4<T>*)e.b.pos, e.b.cell_start, f.nbr_f, f.nbr_b, f.nbr_nf,
@@ -1051,9 +1054,9 @@ __global__ void __launch_bounds__(kBlock) kb_solver_update(char* tab, int ns, in
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) kb_advect(char* tab, int ns) {
-  for_tiles<T, kTileFluid>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
-    TsphObject& f = e.f;
+__global__ void __launch_bounds__(kBlock) kb_advect(const char* __restrict__ tab, int ns) {
+  for_tiles<T, kTileFluid>(tab, ns, [&](const SceneEntry<T>& e, int, Blk blk) {
+    const TsphObject& f = e.f;
     k_advect_body<T>(blk, e.k, f.n, f.capacity, (const V4<T>*)f.pos, (const T*)f.vol,
                      (const T*)f.rest_density, f.cell_start, e.b.cell_start,
                      (const V4<T>*)e.b.pos, f.nbr_f, f.nbr_nf, (V4<T>*)f.vel_adv, (V4<T>*)f.vel,
@@ -1062,9 +1065,9 @@ __global__ void __launch_bounds__(kBlock) kb_advect(char* tab, int ns) {
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) kb_surface_tension(char* tab, int ns) {
-  for_tiles<T, kTileFluid>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
-    TsphObject& f = e.f;
+__global__ void __launch_bounds__(kBlock) kb_surface_tension(const char* __restrict__ tab, int ns) {
+  for_tiles<T, kTileFluid>(tab, ns, [&](const SceneEntry<T>& e, int, Blk blk) {
+    const TsphObject& f = e.f;
     k_surface_tension_body<T>(blk, e.k, f.n, (const V4<T>*)f.pos, (const T*)f.mass, psi_of(e),
                               (const T*)f.rest_density, (const T*)f.volume_frac, f.cell_start,
                               (const V4<T>*)f.normal, (const V4<T>*)f.vel, (V4<T>*)f.acce_adv,
@@ -1073,9 +1076,9 @@ __global__ void __launch_bounds__(kBlock) kb_surface_tension(char* tab, int ns) 
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) kb_tail(char* tab, int ns) {
-  for_tiles<T, kTileFluid>(tab, ns, [&](SceneEntry<T>& e, int, Blk blk) {
-    TsphObject& f = e.f;
+__global__ void __launch_bounds__(kBlock) kb_tail(const char* __restrict__ tab, int ns) {
+  for_tiles<T, kTileFluid>(tab, ns, [&](const SceneEntry<T>& e, int, Blk blk) {
+    const TsphObject& f = e.f;
     k_tail_body<T>(blk, e.k, f.n, 0, (V4<T>*)f.pos, (V4<T>*)f.vel, (const V4<T>*)f.vel_adv,
                    (T*)f.mass, (T*)f.rest_density, (const T*)f.rest_volume, (T*)f.volume_frac,
                    (const T*)f.volume_frac_tmp, f.flag, e.L.partial, e.ctrl);
@@ -1083,16 +1086,16 @@ __global__ void __launch_bounds__(kBlock) kb_tail(char* tab, int ns) {
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) kb_finalize_energy(char* tab, int ns) {
+__global__ void __launch_bounds__(kBlock) kb_finalize_energy(const char* __restrict__ tab, int ns) {
   const int* firstf = (const int*)(tab + batch_entries_bytes<T>(ns)) + kTileFluid * (ns + 1);
-  for_tiles<T, kTileScene>(tab, ns, [&](SceneEntry<T>& e, int s, Blk blk) {
+  for_tiles<T, kTileScene>(tab, ns, [&](const SceneEntry<T>& e, int s, Blk blk) {
     k_finalize_body<T>(blk, e.k, (int)kFinEnergy, e.L.partial, firstf[s + 1] - firstf[s], e.f.n,
                        e.ctrl);
   });
 }
 
 template <class T>
-__global__ void kb_stop_state(char* tab, int ns, int* out) {
+__global__ void kb_stop_state(const char* __restrict__ tab, int ns, int* out) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
   const TsphControl* c = ((SceneEntry<T>*)tab)[s].ctrl;

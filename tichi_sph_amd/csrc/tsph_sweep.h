@@ -82,6 +82,34 @@ __device__ __forceinline__ void for_bound_nbrs(const KP<T>& k, int i, int cap, i
 // whose list overflowed walks the cells with the same in-range test.
 constexpr int kNb = 8;
 
+// One list, software-pipelined: the next batch's list entries are loaded
+// right after this batch's records are requested, so a batch costs one
+// round trip to memory instead of two (index, then record).
+template <class Rec, class Load, class Pair>
+__device__ __forceinline__ void sweep_one_list(int cap, int i, const int* __restrict__ list, int n,
+                                               Load&& load, Pair&& pair) {
+  int j[kNb];
+#pragma unroll
+  for (int u = 0; u < kNb; ++u) j[u] = u < n ? list[(long)u * cap + i] : -1;
+  for (int s0 = 0; s0 < n; s0 += kNb) {
+    Rec d[kNb];
+#pragma unroll
+    for (int u = 0; u < kNb; ++u)
+      if (j[u] >= 0) d[u] = load(j[u]);
+    int jn[kNb];
+#pragma unroll
+    for (int u = 0; u < kNb; ++u) {
+      const int s = s0 + kNb + u;
+      jn[u] = s < n ? list[(long)s * cap + i] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < kNb; ++u)
+      if (j[u] >= 0) pair(d[u]);
+#pragma unroll
+    for (int u = 0; u < kNb; ++u) j[u] = jn[u];
+  }
+}
+
 template <class T, class Rec, class LoadF, class LoadB, class PairF, class PairB>
 __device__ __forceinline__ void sweep_lists(const KP<T>& k, int cap, int i, V4<T> p,
                                             const int* __restrict__ fstart,
@@ -93,30 +121,8 @@ __device__ __forceinline__ void sweep_lists(const KP<T>& k, int cap, int i, V4<T
                                             LoadF&& load_f, LoadB&& load_b, PairF&& pair_f,
                                             PairB&& pair_b) {
   if (nf <= k.list_cap_f && nbb <= k.list_cap_b) {
-    for (int s0 = 0; s0 < nf; s0 += kNb) {
-      int j[kNb];
-      Rec d[kNb];
-#pragma unroll
-      for (int u = 0; u < kNb; ++u) j[u] = s0 + u < nf ? nbr_f[(long)(s0 + u) * cap + i] : -1;
-#pragma unroll
-      for (int u = 0; u < kNb; ++u)
-        if (j[u] >= 0) d[u] = load_f(j[u]);
-#pragma unroll
-      for (int u = 0; u < kNb; ++u)
-        if (j[u] >= 0) pair_f(d[u]);
-    }
-    for (int s0 = 0; s0 < nbb; s0 += kNb) {
-      int j[kNb];
-      Rec d[kNb];
-#pragma unroll
-      for (int u = 0; u < kNb; ++u) j[u] = s0 + u < nbb ? nbr_b[(long)(s0 + u) * cap + i] : -1;
-#pragma unroll
-      for (int u = 0; u < kNb; ++u)
-        if (j[u] >= 0) d[u] = load_b(j[u]);
-#pragma unroll
-      for (int u = 0; u < kNb; ++u)
-        if (j[u] >= 0) pair_b(d[u]);
-    }
+    sweep_one_list<Rec>(cap, i, nbr_f, nf, load_f, pair_f);
+    if (nbb > 0) sweep_one_list<Rec>(cap, i, nbr_b, nbb, load_b, pair_b);
     return;
   }
   int c[3];
