@@ -67,8 +67,13 @@ __device__ __forceinline__ void pass_a_epilogue(const KP<T>& k, int i, const V4<
   if (al < T(1e-4)) al = T(1e-4);
   alpha[i] = al;
   T psi = k.solver == TSPH_VFSPH ? a.cp : a.dn;
-  vol[i] = p.w / psi;   // X / sph_psi, reused by the viscosity and normal sweeps
-  vel_adv[i] = vel[i];  // SPH_vel_2_vel_adv, sph.py:253-256
+  const T Vi = p.w / psi;  // X / sph_psi, reused by the viscosity and normal sweeps
+  vol[i] = Vi;
+  // SPH_vel_2_vel_adv, sph.py:253-256.  The unused 4th lane carries V_i until
+  // the advection sweep, so that sweep gathers one record less per pair; the
+  // Jacobi sweeps only update x, y, z.
+  const V4<T> v = vel[i];
+  vel_adv[i] = V4<T>{v.x, v.y, v.z, Vi};
   if (nf > k.list_cap_f) {
     atomicAdd(&ctrl->overflow_f, 1);
     nf = k.list_cap_f + 1;
@@ -102,16 +107,14 @@ __device__ __forceinline__ void k_pass_a_fluid_body(Blk blk, KP<T> k, int n, int
   node_encode(k, p.x, p.y, p.z, c);
   PassAAcc<T> acc;
   int nf = 0, nbb = 0;
-  walk_cells(k, fstart, c, [&](int j) {
-    const V4<T> q = fpos[j];
+  walk_cells_batched<kPassAU>(k, fstart, c, fpos, [&](int j, const V4<T>& q) {
     if (s_r2(p.x - q.x, p.y - q.y, p.z - q.z) <= k.r2max &&
         acc.add(k, p, q, fmass[j], frvol[j], true)) {
       if (nf < capf) nbr_f[(long)nf * cap + i] = j;
       ++nf;
     }
   });
-  walk_cells(k, bstart, c, [&](int j) {
-    const V4<T> q = bpos[j];
+  walk_cells_batched<kPassAU>(k, bstart, c, bpos, [&](int j, const V4<T>& q) {
     if (s_r2(p.x - q.x, p.y - q.y, p.z - q.z) <= k.r2max &&
         acc.add(k, p, q, bmass[j], brvol[j], false)) {
       if (nbb < capb) nbr_b[(long)nbb * cap + i] = j;
@@ -154,8 +157,7 @@ __device__ __forceinline__ void k_pass_a_bound_body(Blk blk, KP<T> k, int nb, co
   int c[3];
   node_encode(k, p.x, p.y, p.z, c);
   T w = 0, cp = 0, dn = 0, a2 = 0;
-  walk_cells(k, bstart, c, [&](int j) {
-    V4<T> q = bpos[j];
+  walk_cells_batched<kPassAU>(k, bstart, c, bpos, [&](int j, const V4<T>& q) {
     T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
     if (r2 <= k.r2max) {
       T Wr = s_W(k, s_sqrt(r2));
@@ -164,8 +166,7 @@ __device__ __forceinline__ void k_pass_a_bound_body(Blk blk, KP<T> k, int nb, co
       dn = dn + Wr * bmass[j];
     }
   });
-  walk_cells(k, fstart, c, [&](int j) {
-    V4<T> q = fpos[j];
+  walk_cells_batched<kPassAU>(k, fstart, c, fpos, [&](int j, const V4<T>& q) {
     T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
     if (r2 <= k.r2max) {
       T r = s_sqrt(r2);
@@ -409,7 +410,7 @@ __device__ __forceinline__ void k_advect_body(Blk blk, KP<T> k, int n, int cap, 
   auto term = [&](const Rec2<T>& d) {
     const V4<T>& q = d.a;
     const V4<T>& vj = d.b;
-    T Vj = vj.w;  // vol = X / sph_psi rides in the velocity's 4th lane
+    T Vj = vj.w;  // V_j = X_j / sph_psi_j rides in vel_adv's 4th lane (pass A)
     T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     T r2 = dx * dx + dy * dy + dz * dz, r;
     T g = f_grad(k, r2, &r);
@@ -426,13 +427,9 @@ __device__ __forceinline__ void k_advect_body(Blk blk, KP<T> k, int n, int cap, 
   // viscosity and normals are (f,f) only: the boundary list is not visited
   sweep_lists<T, Rec2<T>>(
       k, cap, i, p, fstart, bstart, fpos, bpos, nbr_f, nullptr, nbr_nf[i], 0,
-      [&](int j) {
-        V4<T> v = vadv[j];
-        v.w = vol[j];
-        return Rec2<T>{fpos[j], v};
-      },
+      [&](int j) { return Rec2<T>{fpos[j], vadv[j]}; },
       [&](int j) { return Rec2<T>{}; }, term, [&](const Rec2<T>&) {});
-  vel[i] = vi;  // SPH_vel_adv_2_vel
+  vel[i] = V4<T>{vi.x, vi.y, vi.z, T(0)};  // SPH_vel_adv_2_vel
   acce_adv[i] = V4<T>{ax, ay, az, T(0)};
   normal[i] = V4<T>{nx * k.h, ny * k.h, nz * k.h, T(0)};
 }

@@ -11,10 +11,11 @@
 
 namespace tsph {
 
-// Visit every candidate slot of `cstart`'s object in the cells around c[].
+// Visit the row runs [j0, j1) of `cstart`'s object in the cells around c[]:
+// one run per (dy, dz) row, in template order.
 template <class T, class F>
-__device__ __forceinline__ void walk_cells(const KP<T>& k, const int* __restrict__ cstart,
-                                           const int* c, F&& f) {
+__device__ __forceinline__ void walk_runs(const KP<T>& k, const int* __restrict__ cstart,
+                                          const int* c, F&& f) {
   const int R = k.R;
   const int zr = k.dim == 3 ? R : 0;
 #pragma unroll 1
@@ -26,11 +27,41 @@ __device__ __forceinline__ void walk_cells(const KP<T>& k, const int* __restrict
       long hi = (long)c0 + 2 * R;
       if (hi > k.node_num - 1) hi = k.node_num - 1;
       if (lo > hi) continue;
-      int j0 = cstart[lo], j1 = cstart[hi + 1];
-      for (int j = j0; j < j1; ++j) f(j);
+      f(cstart[lo], cstart[hi + 1]);
     }
   }
 }
+
+// Visit every candidate slot of `cstart`'s object in the cells around c[].
+template <class T, class F>
+__device__ __forceinline__ void walk_cells(const KP<T>& k, const int* __restrict__ cstart,
+                                           const int* c, F&& f) {
+  walk_runs(k, cstart, c, [&](int j0, int j1) {
+    for (int j = j0; j < j1; ++j) f(j);
+  });
+}
+
+// Candidate walk of pass A: the candidates of a run are gathered U at a
+// time, all U position loads in flight together, then f(j, q) runs for each
+// in walk order.  The walk is bound by the latency of these gathers (each
+// trip of the one-at-a-time loop waited a full round trip), not by their
+// number.
+template <int U, class T, class F>
+__device__ __forceinline__ void walk_cells_batched(const KP<T>& k, const int* __restrict__ cstart,
+                                                   const int* c, const V4<T>* __restrict__ pos,
+                                                   F&& f) {
+  walk_runs(k, cstart, c, [&](int j0, int j1) {
+    for (int j = j0; j < j1; j += U) {
+      V4<T> q[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) q[u] = pos[j + u < j1 ? j + u : j];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (j + u < j1) f(j + u, q[u]);
+    }
+  });
+}
+constexpr int kPassAU = 8;
 
 template <class T>
 __device__ __forceinline__ T rest_psi_of(const KP<T>& k, const T* rdens, int i) {
