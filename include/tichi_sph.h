@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define TSPH_ABI_VERSION 1
+#define TSPH_ABI_VERSION 2
 #define TSPH_MAX_PHASE 8
 #define TSPH_MAX_ITER_LOG 256
 
@@ -104,6 +104,8 @@ typedef struct TsphControl {
   double err_div[TSPH_MAX_ITER_LOG];
   double err_incom[TSPH_MAX_ITER_LOG];
   int32_t overflow_f, overflow_b, counter2, pad;
+  uint64_t signal;       /* host address for the early stop signal, or 0 (tsph_solver_begin_signal) */
+  int32_t token, pad2;   /* the value the signal's first word receives */
 } TsphControl;
 
 const char* tsph_last_error(void);
@@ -182,6 +184,28 @@ int tsph_density_alpha(const TsphParams* p, TsphObject* fluid, TsphObject* bound
  * of the while loop.  solve = 0 divergence, 1 density.
  */
 int tsph_solver_begin(const TsphParams* p, TsphControl* ctrl, int32_t solve, void* stream);
+
+/*
+ * tsph_solver_begin plus an early stop signal for the host.  This has no
+ * counterpart in the reference, whose host loop reads the stop flag back
+ * after every iteration (main.py:201-213, 222-234).  `signal` is pinned host
+ * memory mapped for the device (one 8-byte aligned uint64).  Every update
+ * half-step of this solve that runs stores, in one system-scope 64-bit store,
+ * (token << 32) | (2 * it + stop): the iterations done and whether the loop
+ * ends there.  The decision is taken when the update launch starts, so a host
+ * that waits for its token can queue the work that follows the solve while
+ * the last update still runs, instead of draining the stream.  A solve that
+ * runs no iteration writes nothing.  signal = NULL behaves as
+ * tsph_solver_begin.
+ */
+int tsph_solver_begin_signal(const TsphParams* p, TsphControl* ctrl, int32_t solve,
+                             uint64_t* signal, int32_t token, void* stream);
+
+/* Pinned, device-mapped, coherent host memory for the signal above (`bytes`
+ * bytes, zeroed): *host is the address the host reads, *device the one to
+ * pass as `signal`.  Release it with tsph_signal_free(host). */
+int tsph_signal_alloc(size_t bytes, void** host, void** device);
+int tsph_signal_free(void* host);
 
 /*
  * One predicated Jacobi iteration k of a solve: IPPE_adv_psi_init +

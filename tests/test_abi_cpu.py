@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     assert set(decl) == set(_lib.EXPORTED)
     for name in decl:
         assert hasattr(lib, name), name
-    assert lib.tsph_abi_version() == 1
+    assert lib.tsph_abi_version() == _lib.ABI_VERSION == 2
 
 
 def test_struct_layout_matches_header(tmp_path):
@@ -35,7 +35,7 @@ def test_struct_layout_matches_header(tmp_path):
         "TsphParams": ["dim", "list_cap_f", "grid_size", "cs", "list_cap_b", "lb", "phase_rest_density",
                        "part_size", "kernel_sig", "r2_max", "cfl_factor", "compression_threshold"],
         "TsphObject": ["n", "pos", "pressure", "pid", "cell_structured", "cell_start", "nbr_nb", "pid_b"],
-        "TsphControl": ["dt", "it", "stop", "general_flag", "ke", "run_u", "err_div", "err_incom",
+        "TsphControl": ["dt", "it", "stop", "general_flag", "ke", "run_u", "err_div", "err_incom", "signal", "token",
                         "overflow_f", "counter2"],
         "TsphAdjoint": ["xbar", "vbar_out", "nbar", "a1bar", "kappa_next", "tape_normal", "grad"],
     }

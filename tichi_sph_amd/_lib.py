@@ -9,6 +9,7 @@ import os
 
 TSPH_MAX_PHASE = 8
 TSPH_MAX_ITER_LOG = 256
+ABI_VERSION = 2  # TSPH_ABI_VERSION of include/tichi_sph.h
 TSPH_DFSPH = 0
 TSPH_VFSPH = 1
 
@@ -66,6 +67,7 @@ class TsphControl(ctypes.Structure):
         ("err_div", _f64 * TSPH_MAX_ITER_LOG),
         ("err_incom", _f64 * TSPH_MAX_ITER_LOG),
         ("overflow_f", _i32), ("overflow_b", _i32), ("counter2", _i32), ("pad", _i32),
+        ("signal", ctypes.c_uint64), ("token", _i32), ("pad2", _i32),
     ]
 
 
@@ -97,7 +99,8 @@ KERNEL_IDS = {
 EXPORTED = [
     "tsph_last_error", "tsph_abi_version", "tsph_control_bytes", "tsph_workspace_bytes",
     "tsph_sort_object", "tsph_ngrid_build", "tsph_neighbor_lists", "tsph_density_alpha",
-    "tsph_solver_begin", "tsph_solver_iteration", "tsph_solver_psi", "tsph_solver_update", "tsph_advect",
+    "tsph_solver_begin", "tsph_solver_begin_signal", "tsph_signal_alloc", "tsph_signal_free",
+    "tsph_solver_iteration", "tsph_solver_psi", "tsph_solver_update", "tsph_advect",
     "tsph_fbm_round", "tsph_tail",
     "tsph_ref_kernel", "tsph_solver_psi_taped", "tsph_relist", "tsph_adj_begin",
     "tsph_adj_solver_iteration", "tsph_adj_advect", "tsph_adj_density_alpha", "tsph_adj_end",
@@ -145,6 +148,9 @@ def load(path=None):
         "tsph_neighbor_lists": [P, O, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
         "tsph_density_alpha": [P, O, O, C, _vp, sz, _vp],
         "tsph_solver_begin": [P, C, _i32, _vp],
+        "tsph_solver_begin_signal": [P, C, _i32, _vp, _i32, _vp],
+        "tsph_signal_alloc": [ctypes.c_size_t, ctypes.POINTER(_vp), ctypes.POINTER(_vp)],
+        "tsph_signal_free": [_vp],
         "tsph_solver_iteration": [P, O, O, C, _i32, _vp, sz, _vp],
         "tsph_solver_psi": [P, O, O, C, _i32, _vp, sz, _vp],
         "tsph_solver_update": [P, O, O, C, _i32, _vp, sz, _vp],
@@ -175,7 +181,7 @@ def load(path=None):
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = ctypes.c_int
-    if lib.tsph_abi_version() != 1:
+    if lib.tsph_abi_version() != ABI_VERSION:
         raise TsphError("libtichi_sph.so ABI version mismatch")
     if lib.tsph_control_bytes() != ctypes.sizeof(TsphControl):
         raise TsphError("TsphControl layout mismatch between Python and the library")

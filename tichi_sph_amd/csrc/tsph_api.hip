@@ -1,6 +1,8 @@
 // extern "C" entry points declared in include/tichi_sph.h.
 #include <cstdarg>
 
+#include <cstring>
+
 #include "tsph_host.h"
 
 namespace tsph {
@@ -25,7 +27,7 @@ template <class T>
 int density_alpha(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t,
                   hipStream_t);
 template <class T>
-int solver_begin(const TsphParams&, TsphControl*, int, hipStream_t);
+int solver_begin(const TsphParams&, TsphControl*, int, int*, int, hipStream_t);
 template <class T>
 int solver_half(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, int, void*, void*,
                 void*, size_t, hipStream_t);
@@ -200,6 +202,34 @@ const char* tsph_last_error(void) { return g_err.c_str(); }
 int tsph_abi_version(void) { return TSPH_ABI_VERSION; }
 size_t tsph_control_bytes(void) { return sizeof(TsphControl); }
 
+int tsph_signal_alloc(size_t bytes, void** host, void** device) {
+  TSPH_REQUIRE(host && device && bytes > 0, TSPH_EINVAL, "tsph_signal_alloc: bad arguments");
+  *host = nullptr;
+  *device = nullptr;
+  void* h = nullptr;
+  hipError_t e = hipHostMalloc(&h, bytes, hipHostMallocMapped | hipHostMallocCoherent);
+  TSPH_REQUIRE(e == hipSuccess, TSPH_EHIP, "tsph_signal_alloc: hipHostMalloc: %s",
+               hipGetErrorString(e));
+  memset(h, 0, bytes);
+  void* d = nullptr;
+  e = hipHostGetDevicePointer(&d, h, 0);
+  if (e != hipSuccess) {
+    (void)hipHostFree(h);
+    TSPH_REQUIRE(false, TSPH_EHIP, "tsph_signal_alloc: hipHostGetDevicePointer: %s",
+                 hipGetErrorString(e));
+  }
+  *host = h;
+  *device = d;
+  return 0;
+}
+
+int tsph_signal_free(void* host) {
+  if (!host) return 0;
+  hipError_t e = hipHostFree(host);
+  TSPH_REQUIRE(e == hipSuccess, TSPH_EHIP, "tsph_signal_free: %s", hipGetErrorString(e));
+  return 0;
+}
+
 size_t tsph_workspace_bytes(const TsphParams* p, int32_t capacity) {
   if (!p) return 0;
   return ws_bytes_needed(capacity, p->node_num);
@@ -244,7 +274,14 @@ int tsph_density_alpha(const TsphParams* p, TsphObject* f, TsphObject* b, TsphCo
 
 int tsph_solver_begin(const TsphParams* p, TsphControl* ctrl, int32_t solve, void* stream) {
   if (int rc = check_params(p)) return rc;
-  return DISPATCH(p, solver_begin, *p, ctrl, solve, (hipStream_t)stream);
+  return DISPATCH(p, solver_begin, *p, ctrl, solve, (int*)nullptr, 0, (hipStream_t)stream);
+}
+
+int tsph_solver_begin_signal(const TsphParams* p, TsphControl* ctrl, int32_t solve,
+                             uint64_t* signal, int32_t token, void* stream) {
+  if (int rc = check_params(p)) return rc;
+  return DISPATCH(p, solver_begin, *p, ctrl, solve, (int*)signal, (int)token,
+                  (hipStream_t)stream);
 }
 
 static int solver_entry(const TsphParams* p, TsphObject* f, TsphObject* b, TsphControl* ctrl,

@@ -125,6 +125,8 @@ struct WsLayout {
   int* tmp_key;        // [cap]
   int* cu_count;       // [2*node_num + 2] reference grid (cell, uid) counts
   int* cu_start;       // [2*node_num + 3]
+  void* krec;          // [cap] V4 of the step's precision: position and kappa of
+                       // each fluid particle in the current Jacobi iteration
 };
 
 inline size_t ws_bytes_needed(long cap, long node_num) {
@@ -132,7 +134,7 @@ inline size_t ws_bytes_needed(long cap, long node_num) {
   long lenref = 2 * node_num + 2;
   return seg(sizeof(double) * 2 * maxblk) + seg(sizeof(int) * (scan_blocks(lenref + 1) + 8)) +
          5 * seg(sizeof(int) * cap) + seg(sizeof(int) * (lenref + 1)) +
-         seg(sizeof(int) * (lenref + 2)) + 4096;
+         seg(sizeof(int) * (lenref + 2)) + seg(4 * sizeof(double) * cap) + 4096;
 }
 
 inline bool carve_ws(void* ws, size_t bytes, long cap, long node_num, WsLayout* L) {
@@ -148,6 +150,7 @@ inline bool carve_ws(void* ws, size_t bytes, long cap, long node_num, WsLayout* 
   L->tmp_key = c.take<int>(cap);
   L->cu_count = c.take<int>(lenref + 1);
   L->cu_start = c.take<int>(lenref + 2);
+  L->krec = c.take<double>(4 * cap);
   return c.ok;
 }
 

@@ -222,7 +222,8 @@ __global__ void k_step_reset(TsphControl* ctrl) { step_reset_body(ctrl); }
 // Solver stop state (main.py:201-213 / 222-234).
 // ---------------------------------------------------------------------------
 template <class T>
-__device__ __forceinline__ void solver_begin_body(const KP<T>& k, TsphControl* ctrl, int solve) {
+__device__ __forceinline__ void solver_begin_body(const KP<T>& k, TsphControl* ctrl, int solve,
+                                                  int* signal = nullptr, int token = 0) {
   int t = threadIdx.x;
   T thr = solve == 0 ? k.div_thr : k.comp_thr;
   T comp = (T)ctrl->compression;
@@ -235,11 +236,13 @@ __device__ __forceinline__ void solver_begin_body(const KP<T>& k, TsphControl* c
     ctrl->stop = enter ? 0 : 1;
     if (solve == 0) ctrl->div_iter = 0;
     else ctrl->incom_iter = 0;
+    ctrl->signal = (uint64_t)(uintptr_t)signal;
+    ctrl->token = token;
   }
 }
 template <class T>
-__global__ void k_solver_begin(KP<T> k, TsphControl* ctrl, int solve) {
-  solver_begin_body(k, ctrl, solve);
+__global__ void k_solver_begin(KP<T> k, TsphControl* ctrl, int solve, int* signal, int token) {
+  solver_begin_body(k, ctrl, solve, signal, token);
 }
 
 // Jacobi half-step P: IPPE_adv_psi_init + IPPE_adv_psi (f,f)(f,b) +
@@ -252,8 +255,8 @@ __device__ __forceinline__ void k_solver_psi_body(Blk blk, KP<T> k, int n, int c
                  const int* __restrict__ bstart, const int* __restrict__ nbr_f,
                  const int* __restrict__ nbr_b, const int* __restrict__ nbr_nf,
                  const int* __restrict__ nbr_nb, T* __restrict__ psi_adv, T* __restrict__ kappa,
-                 V4<T>* __restrict__ tape_v, T* __restrict__ tape_k, double* __restrict__ partial,
-                 TsphControl* ctrl) {
+                 V4<T>* __restrict__ krec, V4<T>* __restrict__ tape_v, T* __restrict__ tape_k,
+                 double* __restrict__ partial, TsphControl* ctrl) {
   __shared__ double lds[kWaves];
   if (!ctrl->run_u[kk]) return;
   const int i = xcd_block(blk.b, blk.nb) * kBlock + threadIdx.x;
@@ -282,6 +285,7 @@ __device__ __forceinline__ void k_solver_psi_body(Blk blk, KP<T> k, int n, int c
     psi_adv[i] = psi;
     const T kap = psi / alpha[i];
     kappa[i] = kap;
+    krec[i] = V4<T>{p.x, p.y, p.z, kap};  // the record the update sweep gathers
     if (tape_v) {  // record the iteration for the adjoint
       tape_v[i] = va;
       tape_k[i] = kap;
@@ -301,17 +305,25 @@ __global__ void __launch_bounds__(kBlock)
                  const int* __restrict__ bstart, const int* __restrict__ nbr_f,
                  const int* __restrict__ nbr_b, const int* __restrict__ nbr_nf,
                  const int* __restrict__ nbr_nb, T* __restrict__ psi_adv, T* __restrict__ kappa,
-                 V4<T>* __restrict__ tape_v, T* __restrict__ tape_k, double* __restrict__ partial,
-                 TsphControl* ctrl) {
-  k_solver_psi_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, n, cap, kk, fpos, vadv, psi_rho, rdens, alpha, fstart, bpos, bstart, nbr_f, nbr_b, nbr_nf, nbr_nb, psi_adv, kappa, tape_v, tape_k, partial, ctrl);
+                 V4<T>* __restrict__ krec, V4<T>* __restrict__ tape_v, T* __restrict__ tape_k,
+                 double* __restrict__ partial, TsphControl* ctrl) {
+  k_solver_psi_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, n, cap, kk, fpos, vadv, psi_rho, rdens, alpha, fstart, bpos, bstart, nbr_f, nbr_b, nbr_nf, nbr_nb, psi_adv, kappa, krec, tape_v, tape_k, partial, ctrl);
 }
 
 // Jacobi half-step U: IPPE_update_vel_adv (f,f)(f,b) (sph.py:229-244), with
 // kappa = psi_adv / alpha formed once per particle.  Block 0 also finishes
 // iteration kk's compression and takes the loop decision for iteration kk+1.
+//
+// The pair coefficient of a fluid neighbour is kappa_i X_j + kappa_j X_i, of
+// a boundary neighbour kappa_i X_j (boundary kappa is never written: 0).
+// Summed over all neighbours, the kappa_i X_j halves are kappa_i times
+// alpha_1 = sum_j X_j grad W_ij of pass A (positions and X do not change
+// within the step), so the sweep only visits the fluid list, gathering the
+// one record (x_j, kappa_j) that the psi half-step wrote.
 template <class T>
 __device__ __forceinline__ void k_solver_update_body(Blk blk, KP<T> k, int n, int cap, int kk, const V4<T>* __restrict__ fpos,
                     const T* __restrict__ fmass, const T* __restrict__ kappa,
+                    const V4<T>* __restrict__ krec, const V4<T>* __restrict__ alpha_1,
                     const int* __restrict__ fstart, const V4<T>* __restrict__ bpos,
                     const int* __restrict__ bstart, const int* __restrict__ nbr_f,
                     const int* __restrict__ nbr_b, const int* __restrict__ nbr_nf,
@@ -339,6 +351,11 @@ __device__ __forceinline__ void k_solver_update_body(Blk blk, KP<T> k, int n, in
       bool run = it <= k.iter_max && cont && it < TSPH_MAX_ITER_LOG;
       if (it < TSPH_MAX_ITER_LOG) ctrl->run_u[it] = run ? 1 : 0;
       ctrl->stop = run ? 0 : 1;
+      if (ctrl->signal) {  // early stop signal to the host: one 64-bit system-scope store
+        const uint64_t w = ((uint64_t)(uint32_t)ctrl->token << 32) | (uint32_t)(2 * it + (run ? 0 : 1));
+        __hip_atomic_store((uint64_t*)(uintptr_t)ctrl->signal, w, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+      }
     }
   }
   const int i = xcd_block(blk.b, blk.nb) * kBlock + threadIdx.x;
@@ -349,29 +366,25 @@ __device__ __forceinline__ void k_solver_update_body(Blk blk, KP<T> k, int n, in
   const T ki = kappa[i];
   const T Xi = p.w;
   const T c = -(T(1) / dt) / fmass[i];
-  auto term = [&](const V4<T>& q) {  // q.w carries kappa_j * X_i + kappa_i * X_j
+  const V4<T> A = alpha_1[i];
+  const T cA = c * ki;
+  va.x += cA * A.x;
+  va.y += cA * A.y;
+  va.z += cA * A.z;
+  const T cX = c * Xi;
+  auto term = [&](const V4<T>& q) {  // q = (x_j, kappa_j)
     T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     T r2 = dx * dx + dy * dy + dz * dz, r;
     T g = f_grad(k, r2, &r);
-    T s = c * q.w * g;
+    T s = cX * q.w * g;
     va.x += s * dx;
     va.y += s * dy;
     va.z += s * dz;
   };
   sweep_lists<T, V4<T>>(
-      k, cap, i, p, fstart, bstart, fpos, bpos, nbr_f, nbr_b, nbr_nf[i], nbr_nb[i],
-      [&](int j) {
-        V4<T> q = fpos[j];
-        q.w = ki * q.w + kappa[j] * Xi;
-        return q;
-      },
-      // boundary psi_adv is never written (0), so only the kappa_i term remains
-      [&](int j) {
-        V4<T> q = bpos[j];
-        q.w = ki * q.w;
-        return q;
-      },
-      term, term);
+      k, cap, i, p, fstart, bstart, fpos, bpos, nbr_f, nullptr, nbr_nf[i], 0,
+      [&](int j) { return krec[j]; }, [&](int) { return V4<T>{}; }, term,
+      [&](const V4<T>&) {});
   vadv[i] = va;
 }
 
@@ -379,12 +392,13 @@ template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_solver_update(KP<T> k, int n, int cap, int kk, const V4<T>* __restrict__ fpos,
                     const T* __restrict__ fmass, const T* __restrict__ kappa,
+                    const V4<T>* __restrict__ krec, const V4<T>* __restrict__ alpha_1,
                     const int* __restrict__ fstart, const V4<T>* __restrict__ bpos,
                     const int* __restrict__ bstart, const int* __restrict__ nbr_f,
                     const int* __restrict__ nbr_b, const int* __restrict__ nbr_nf,
                     const int* __restrict__ nbr_nb, V4<T>* __restrict__ vadv,
                     const double* __restrict__ partial, TsphControl* ctrl) {
-  k_solver_update_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, n, cap, kk, fpos, fmass, kappa, fstart, bpos, bstart, nbr_f, nbr_b, nbr_nf, nbr_nb, vadv, partial, ctrl);
+  k_solver_update_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, n, cap, kk, fpos, fmass, kappa, krec, alpha_1, fstart, bpos, bstart, nbr_f, nbr_b, nbr_nf, nbr_nb, vadv, partial, ctrl);
 }
 
 // ---------------------------------------------------------------------------
@@ -734,9 +748,11 @@ int relist(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl* ctr
 }
 
 template <class T>
-int solver_begin(const TsphParams& prm, TsphControl* ctrl, int solve, hipStream_t st) {
+int solver_begin(const TsphParams& prm, TsphControl* ctrl, int solve, int* signal, int token,
+                 hipStream_t st) {
   KP<T> k = make_kp<T>(prm);
-  hipLaunchKernelGGL(k_solver_begin<T>, dim3(1), dim3(kBlock), 0, st, k, ctrl, solve);
+  hipLaunchKernelGGL(k_solver_begin<T>, dim3(1), dim3(kBlock), 0, st, k, ctrl, solve, signal,
+                     token);
   TSPH_LAUNCH_CHECK("solver_begin");
   return 0;
 }
@@ -756,12 +772,12 @@ int solver_half(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl
                        (const V4<T>*)f->pos, (const V4<T>*)f->vel_adv, psi,
                        (const T*)f->rest_density, (const T*)f->alpha, f->cell_start,
                        (const V4<T>*)b->pos, b->cell_start, f->nbr_f, f->nbr_b, f->nbr_nf,
-                       f->nbr_nb, (T*)f->psi_adv, (T*)f->kappa, (V4<T>*)tape_v, (T*)tape_k,
-                       L.partial, ctrl);
+                       f->nbr_nb, (T*)f->psi_adv, (T*)f->kappa, (V4<T>*)L.krec, (V4<T>*)tape_v,
+                       (T*)tape_k, L.partial, ctrl);
   if (half & 2)
     hipLaunchKernelGGL(k_solver_update<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
                        kk, (const V4<T>*)f->pos, (const T*)f->mass, (const T*)f->kappa,
-                       f->cell_start, (const V4<T>*)b->pos, b->cell_start, f->nbr_f, f->nbr_b,
+                       (const V4<T>*)L.krec, (const V4<T>*)f->alpha_1, f->cell_start, (const V4<T>*)b->pos, b->cell_start, f->nbr_f, f->nbr_b,
                        f->nbr_nf, f->nbr_nb, (V4<T>*)f->vel_adv, L.partial, ctrl);
   TSPH_LAUNCH_CHECK("solver_iteration");
   return 0;
@@ -1035,7 +1051,7 @@ __global__ void __launch_bounds__(kBlock) kb_solver_psi(const char* __restrict__
                          (const V4<T>*)f.vel_adv, psi_of(e), (const T*)f.rest_density,
                          (const T*)f.alpha, f.cell_start, (const V4<T>*)e.b.pos, e.b.cell_start,
                          f.nbr_f, f.nbr_b, f.nbr_nf, f.nbr_nb, (T*)f.psi_adv, (T*)f.kappa,
-                         (V4<T>*)nullptr, (T*)nullptr, e.L.partial, e.ctrl);
+                         (V4<T>*)e.L.krec, (V4<T>*)nullptr, (T*)nullptr, e.L.partial, e.ctrl);
   });
 }
 
@@ -1044,7 +1060,8 @@ __global__ void __launch_bounds__(kBlock) kb_solver_update(const char* __restric
   for_tiles<T, kTileFluid>(tab, ns, [&](const SceneEntry<T>& e, int, Blk blk) {
     const TsphObject& f = e.f;
     k_solver_update_body<T>(blk, e.k, f.n, f.capacity, kk, (const V4<T>*)f.pos,
-                            (const T*)f.mass, (const T*)f.kappa, f.cell_start,
+                            (const T*)f.mass, (const T*)f.kappa, (const V4<T>*)e.L.krec,
+                            (const V4<T>*)f.alpha_1, f.cell_start,
                             (const V4<T>*)e.b.pos, e.b.cell_start, f.nbr_f, f.nbr_b, f.nbr_nf,
                             f.nbr_nb, (V4<T>*)f.vel_adv, e.L.partial, e.ctrl);
   });
@@ -1174,8 +1191,8 @@ template int relist<float>(const TsphParams&, TsphObject*, TsphObject*, TsphCont
 template int relist<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template int density_alpha<float>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template int density_alpha<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
-template int solver_begin<float>(const TsphParams&, TsphControl*, int, hipStream_t);
-template int solver_begin<double>(const TsphParams&, TsphControl*, int, hipStream_t);
+template int solver_begin<float>(const TsphParams&, TsphControl*, int, int*, int, hipStream_t);
+template int solver_begin<double>(const TsphParams&, TsphControl*, int, int*, int, hipStream_t);
 template int solver_half<float>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, int, void*, void*, void*, size_t, hipStream_t);
 template int solver_half<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, int, void*, void*, void*, size_t, hipStream_t);
 template int advect<float>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);

@@ -13,6 +13,7 @@ drop-in use and as a parity anchor.
 """
 import ctypes
 import json
+import sys
 import warnings
 
 import numpy as np
@@ -138,6 +139,16 @@ class Simulation:
         pin = self.device.type == "cuda"
         self._ctrl_host = torch.zeros(self.ctrl.numel(), dtype=torch.uint8, pin_memory=pin)
         self._stop_host = torch.zeros(16, dtype=torch.uint8, pin_memory=pin)
+        # early stop signal of the Jacobi solves (tsph_solver_begin_signal):
+        # one uint64 of pinned, device-mapped, coherent host memory
+        self._token = 0
+        self._signal_host = self._signal_dev = None
+        self._signal_np = np.zeros(1, dtype=np.uint64)
+        if pin:
+            h, d = ctypes.c_void_p(), ctypes.c_void_p()
+            _lib.check(self.lib.tsph_signal_alloc(8, ctypes.byref(h), ctypes.byref(d)), "tsph_signal_alloc")
+            self._signal_host, self._signal_dev = h, d
+            self._signal_np = np.ctypeslib.as_array(ctypes.cast(h, ctypes.POINTER(ctypes.c_uint64)), shape=(1,))
         self.ngrid = Ngrid(self)
         self.bound_dirty = True
         self.pred = [max(config.iter_threshold_min, 1), max(config.iter_threshold_min, 1)]
@@ -202,6 +213,32 @@ class Simulation:
     def _write_ctrl(self, raw):
         buf = torch.frombuffer(bytearray(bytes(raw)), dtype=torch.uint8)
         self.ctrl.copy_(buf.to(self.device))
+
+    def __del__(self):
+        h = getattr(self, "_signal_host", None)
+        if h is not None and not sys.is_finalizing():
+            self.lib.tsph_signal_free(h)
+            self._signal_host = None
+
+    def _wait_signal(self, token, launched):
+        """(it, stop) of the running solve once its update half-steps have
+        decided on the last of the `launched` iterations, read from the early
+        signal without draining the stream; falls back to the control block
+        once the stream has drained (a solve that ran no iteration writes no
+        signal)."""
+        a = self._signal_np
+        stream = torch.cuda.current_stream(self.device)
+        spins = 0
+        while True:
+            v = int(a[0])
+            if v >> 32 == token:
+                w = v & 0xFFFFFFFF
+                if w & 1 or (w >> 1) >= launched:
+                    return w >> 1, w & 1
+            spins += 1
+            if spins % 64 == 0 and stream.query():
+                it, stop, _ = self._stop_state()
+                return it, stop
 
     def _stop_state(self):
         # it, stop are the first two int32 after two doubles
@@ -353,7 +390,11 @@ class Simulation:
         ws, wsb = self._ws()
         st = self.stream_ptr()
         f, b = ctypes.byref(self.fluid.struct()), ctypes.byref(self.bound.struct())
-        _lib.check(self.lib.tsph_solver_begin(P, self.ctrl_ptr(), solve, st), "tsph_solver_begin")
+        self._token = self._token % 0x7FFFFFFF + 1
+        token = self._token
+        _lib.check(self.lib.tsph_solver_begin_signal(P, self.ctrl_ptr(), solve,
+                                                     self._signal_dev, token, st),
+                   "tsph_solver_begin_signal")
         kmax = (self.config.iter_threshold_max if force is None else force) + 1
         if force == 0:
             kmax = 0
@@ -373,7 +414,7 @@ class Simulation:
                                ctypes.c_void_p(tv.data_ptr()), ctypes.c_void_p(tk.data_ptr()), ws, wsb, st)
                 self._call("solver_update", self.lib.tsph_solver_update, P, f, b, self.ctrl_ptr(), k, ws, wsb, st)
                 k += 1
-            it, stop, _ = self._stop_state()
+            it, stop = self._wait_signal(token, k)
             if stop:
                 break
             chunk = 1
