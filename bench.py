@@ -10,7 +10,11 @@ inputs resident in HBM.  Prints one JSON line.
 
 Roofline: algorithmic bytes of the dominant kernel per launch (per-particle
 figures of the minimal-pass schedule) divided by its average launch duration
-from HIP events recorded on the launch stream inside the timed region.
+from HIP events recorded on the launch stream inside the timed region.  Only
+that kernel group's launches, on every 4th timed step, carry events (an event
+pair on every launch costs about 90 us per step); the per-group breakdown
+(`kernel_ms_per_step`) and the choice of the dominant group come from an
+untimed pass of the same steps with every launch timed.
 `traffic` is the dominant kernel's HBM bytes per launch from FETCH_SIZE and
 WRITE_SIZE counter runs, each scaled by what the same counter reads for the
 copy kernel of --copy-calibration; the figures are read from
@@ -77,11 +81,17 @@ def copy_calibration(mib, reps=5):
                       "kernel": "vectorized_elementwise_kernel (MulFunctor)"}), flush=True)
 
 
+TAPE_BUDGET = 8 << 30  # bytes of HBM for kept segment tapes in the differentiable legs
+
+
 def adjoint_leg(scen, steps, every):
     """fwd vs fwd+adjoint on the same workload (fixed dt, which the CFL
     condition never lowers here): `steps` plain forward steps, then a
     differentiable rollout of `steps` steps (checkpoint every `every`) with
-    the backward pass of a position loss; wall time of each, ms per step."""
+    the backward pass of a position loss; wall time of each, ms per step.
+    `ratio` keeps earlier segments' tapes within TAPE_BUDGET bytes;
+    `ratio_replay_all` replays every segment but the last from its checkpoint
+    (tape budget 0).  Both give bit-identical gradients."""
     import torch
     from tichi_sph_amd import load_simulation
     from tichi_sph_amd.adjoint import DiffRunner, rollout
@@ -95,18 +105,18 @@ def adjoint_leg(scen, steps, every):
     x_init = r.positions_ref().clone()
     mu0, gam0 = sim.params.dynamic_viscosity, sim.params.surface_tension_gamma
 
-    def diff_run(k):
+    def diff_run(k, budget):
         r.restore(st0)
         v = torch.zeros(nf, d, dtype=dt_, device=dev, requires_grad=True)
         mu = torch.tensor(mu0, dtype=dt_, device=dev, requires_grad=True)
         # the plain DFSPH workload has no surface tension (gamma = 0): its
         # gradients are those w.r.t. the initial velocity field and viscosity
         gam = torch.tensor(gam0, dtype=dt_, device=dev, requires_grad=gam0 != 0.0)
-        x, _ = rollout(r, v, mu, gam, k, every)
+        x, _ = rollout(r, v, mu, gam, k, every, tape_budget=budget)
         ((x - x_init) ** 2).sum(1).mean().backward()
         return v, mu, gam
 
-    diff_run(2)  # warm-up (allocator, first launches)
+    diff_run(2, TAPE_BUDGET)  # warm-up (allocator, first launches)
     r.restore(st0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -114,24 +124,31 @@ def adjoint_leg(scen, steps, every):
         r.step()
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
+
+    def timed(budget):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = diff_run(steps, budget)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, out
+
+    t_adj, (v, mu, gam) = timed(TAPE_BUDGET)
+    t_rep, (v2, mu2, _) = timed(0)
+    same = bool(torch.equal(v.grad, v2.grad) and float(mu.grad) == float(mu2.grad))
     from tichi_sph_amd import EventTimer
-    sim.timer = EventTimer()
-    diff_run(2)  # sizes the event pool
-    sim.timer = EventTimer(sim.timer.used * steps // 2 + 512)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    v, mu, gam = diff_run(steps)
-    torch.cuda.synchronize()
-    t_adj = time.perf_counter() - t0
+    sim.timer = EventTimer()  # per-call breakdown from a separate, untimed rollout
+    diff_run(steps, TAPE_BUDGET)
     per = sim.timer.per_name()
     sim.timer = None
     r.restore(st0)
     tp = r.step(tape=True)
     tape_mb = tp.nbytes() / 1e6
     del tp
-    ms_f, ms_a = t_fwd / steps * 1e3, t_adj / steps * 1e3
-    return {"steps": steps, "checkpoint_every": every, "ms_per_step_fwd": round(ms_f, 4),
-            "ms_per_step_fwd_adj": round(ms_a, 4), "ratio": round(ms_a / ms_f, 3),
+    ms_f, ms_a, ms_r = t_fwd / steps * 1e3, t_adj / steps * 1e3, t_rep / steps * 1e3
+    return {"steps": steps, "checkpoint_every": every, "tape_budget_bytes": TAPE_BUDGET,
+            "ms_per_step_fwd": round(ms_f, 4), "ms_per_step_fwd_adj": round(ms_a, 4),
+            "ratio": round(ms_a / ms_f, 3), "ms_per_step_fwd_adj_replay_all": round(ms_r, 4),
+            "ratio_replay_all": round(ms_r / ms_f, 3), "gradients_identical": same,
             "value_fwd_adj": round(nf * steps / t_adj, 1),
             "grads": ["initial velocity", "dynamic_viscosity"] + (["surface_tension_gamma"] if gam0 else []),
             "grad_finite": bool(torch.isfinite(v.grad).all().item() and math.isfinite(float(mu.grad))),
@@ -144,7 +161,9 @@ DIFF_WORKLOAD = "diff_3d_200k"
 
 def differentiable_leg(scen, steps=200, every=20):
     """The differentiable dam break: 200k fluid particles, `steps` steps
-    checkpointed every `every`; loss = mean |x - x_target|^2 with the target
+    checkpointed every `every` (tapes kept within TAPE_BUDGET, and, for
+    `ratio_replay_all`, none kept but the last segment's); loss =
+    mean |x - x_target|^2 with the target
     cloud from a forward run at mu = 0.02, gamma = 0.05; gradients w.r.t.
     mu, gamma and the initial velocity field, from mu = 0.01, gamma = 0.07
     and v0 ~ N(0, 0.01^2) per component (numpy default_rng(0))."""
@@ -175,24 +194,39 @@ def differentiable_leg(scen, steps=200, every=20):
     forward(0.01, 0.07)
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
-    r.restore(st0)
-    v = v0.clone().requires_grad_(True)
-    mu = torch.tensor(0.01, dtype=dt_, device=dev, requires_grad=True)
-    gam = torch.tensor(0.07, dtype=dt_, device=dev, requires_grad=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    x, _ = rollout(r, v, mu, gam, steps, every)
-    loss = ((x - target) ** 2).sum(1).mean()
-    loss.backward()
-    torch.cuda.synchronize()
-    t_adj = time.perf_counter() - t0
+    def diff_run(budget):
+        r.restore(st0)
+        v = v0.clone().requires_grad_(True)
+        mu = torch.tensor(0.01, dtype=dt_, device=dev, requires_grad=True)
+        gam = torch.tensor(0.07, dtype=dt_, device=dev, requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, _ = rollout(r, v, mu, gam, steps, every, tape_budget=budget)
+        loss = ((x - target) ** 2).sum(1).mean()
+        loss.backward()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, loss, v, mu, gam
+
+    t_adj, loss, v, mu, gam = diff_run(TAPE_BUDGET)
     its = r.iter_log
+    t_rep, _, v2, mu2, gam2 = diff_run(0)
+    same = bool(torch.equal(v.grad, v2.grad) and float(mu.grad) == float(mu2.grad)
+                and float(gam.grad) == float(gam2.grad))
+    g_mu, g_gam, g_v = float(mu.grad), float(gam.grad), float(v.grad.double().norm())
+    from tichi_sph_amd import EventTimer
+    sim.timer = EventTimer()  # per-call breakdown of forward + adjoint from a separate, untimed rollout
+    diff_run(TAPE_BUDGET)
+    per = sim.timer.per_name()
+    sim.timer = None
     return {"workload": DIFF_WORKLOAD, "fluid_particles": nf, "steps": steps, "checkpoint_every": every,
+            "tape_budget_bytes": TAPE_BUDGET,
             "ms_per_step_fwd": round(t_fwd / steps * 1e3, 4), "ms_per_step_fwd_adj": round(t_adj / steps * 1e3, 4),
-            "ratio": round(t_adj / t_fwd, 3), "loss": float(loss.detach()),
-            "grad_mu": float(mu.grad), "grad_gamma": float(gam.grad),
-            "grad_v0_norm": float(v.grad.double().norm()),
-            "mean_iterations": [sum(a for a, _ in its) / len(its), sum(b for _, b in its) / len(its)]}
+            "ratio": round(t_adj / t_fwd, 3), "ms_per_step_fwd_adj_replay_all": round(t_rep / steps * 1e3, 4),
+            "ratio_replay_all": round(t_rep / t_fwd, 3), "gradients_identical": same,
+            "loss": float(loss.detach()),
+            "grad_mu": g_mu, "grad_gamma": g_gam, "grad_v0_norm": g_v,
+            "mean_iterations": [sum(a for a, _ in its) / len(its), sum(b for _, b in its) / len(its)],
+            "kernel_ms_per_step": {k: round(x[0] / steps, 4) for k, x in sorted(per.items())}}
 
 
 def batched_leg(steps=20, n_scenes=64):
@@ -310,28 +344,40 @@ def main():
     sim = load_simulation(os.path.join(scen, WORKLOAD + ".config.json"), os.path.join(scen, WORKLOAD + ".json"))
     nf, nb, nc = sim.fluid.part_num, sim.bound.part_num, sim.config.node_num
     from tichi_sph_amd import EventTimer
-    for _ in range(args.warmup):
+    # warm-up: at least `warmup` steps and 2 s, so that clocks and caches of a
+    # fresh device have settled before anything is timed
+    t0 = time.perf_counter()
+    n = 0
+    while n < args.warmup or (time.perf_counter() - t0 < 2.0 and n < 20 * args.warmup + 100):
         sim.sph_step()
+        n += 1
     torch.cuda.synchronize()
+    breakdown, dom = {}, None
     if not args.no_timers:
+        prof_steps = max(1, min(10, args.steps))
         sim.timer = EventTimer()
-        sim.sph_step()  # counts the timed calls of one step
-        sim.timer = EventTimer(sim.timer.used * (args.steps + 2) + 256)
+        for _ in range(prof_steps):
+            sim.sph_step()
+        breakdown = {k: v[0] / prof_steps for k, v in sim.timer.per_name().items()}
+        dom = max((k for k in breakdown if k in ALG_BYTES), key=lambda k: breakdown[k])
+    tdom = EventTimer(16 * (args.steps + 2) + 256, only={dom}) if dom else None
     iters = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for s_ in range(args.steps):
+        # the dominant group's launches of every 4th step carry events
+        sim.timer = tdom if s_ % 4 == 0 else None
         iters.append(sim.sph_step())
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = wall / args.steps * 1e3
     value = nf * args.steps / wall
 
-    per = sim.timer.per_name() if sim.timer else {}
+    per = tdom.per_name() if tdom else {}
+    sim.timer = None
     I_mean = sum(a + b for a, b in iters) / len(iters)
     roof = None
-    if per:
-        dom = max((k for k in per if k in ALG_BYTES), key=lambda k: per[k][0])
+    if dom in per:
         avg_ms = per[dom][0] / per[dom][1]
         alg = ALG_BYTES[dom](nf, nb)
         ach = alg / (avg_ms * 1e-3) / 1e9
@@ -353,7 +399,7 @@ def main():
         "iterations": {"mean_div": sum(a for a, _ in iters) / len(iters),
                        "mean_incom": sum(b for _, b in iters) / len(iters),
                        "max_div": max(a for a, _ in iters), "max_incom": max(b for _, b in iters)},
-        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in sorted(per.items())},
+        "kernel_ms_per_step": {k: round(v, 4) for k, v in sorted(breakdown.items())},
     }
     if not args.no_adjoint:
         del sim

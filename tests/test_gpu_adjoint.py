@@ -191,22 +191,26 @@ def test_position_adjoint_matches_fd_fp64():
 
 
 def test_checkpoint_interval_does_not_change_gradient():
-    """Replays are deterministic: every = 1 and every = steps give
+    """Replays are deterministic: every = 1 and every = steps, and tape
+    budgets that keep none, some or all of the earlier segments' tapes, give
     bit-identical gradients."""
-    name, steps = "dambreak_3d_8k", 4
+    name, steps = "dambreak_3d_8k", 6
     r, st0, v0, target, rollout = setup_case(name, steps, precision=4)
+    r.restore(st0)
+    one = r.step(tape=True).nbytes()
     out = []
-    for every in (1, steps):
+    for every, budget in ((1, 0), (steps, 0), (2, 0), (2, int(2.5 * one)), (2, 1 << 40), (1, 1 << 40)):
         r.restore(st0)
         mu = torch.tensor(0.01, dtype=torch.float32, device=v0.device, requires_grad=True)
         gam = torch.tensor(0.07, dtype=torch.float32, device=v0.device, requires_grad=True)
         v = v0.clone().requires_grad_(True)
-        x, _ = rollout(r, v, mu, gam, steps, every)
+        x, _ = rollout(r, v, mu, gam, steps, every, tape_budget=budget)
         loss_of(x, target).backward()
         out.append((float(mu.grad), float(gam.grad), v.grad.clone(), x.detach().clone()))
-    assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
-    assert torch.equal(out[0][2], out[1][2])
-    assert torch.equal(out[0][3], out[1][3])
+    for o in out[1:]:
+        assert out[0][0] == o[0] and out[0][1] == o[1]
+        assert torch.equal(out[0][2], o[2])
+        assert torch.equal(out[0][3], o[3])
 
 
 def test_taped_step_is_bit_identical_to_plain_step():

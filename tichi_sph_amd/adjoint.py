@@ -13,9 +13,15 @@ W'(h) = W''(h) = 0) and the derivative is exact.
 Memory.  The forward pass keeps checkpoints: the persistent state (pos,
 vel, mass, rest density, rest volume, volume fractions, pid and the control
 block) every `every` steps, and the tapes of the last segment, which the
-backward pass reverses first.  The backward pass replays every other
-segment from its checkpoint while recording a tape per step, then reverses
-the segment step by step.  A tape holds the sorted positions, masses, rest
+backward pass reverses first.  With a tape budget (`tape_budget` bytes) it
+also keeps the tapes of the most recent earlier segments whose tapes fit in
+the budget, dropping the oldest taped segment whenever the total exceeds it.
+The backward pass replays every segment whose tapes were not kept from its
+checkpoint while recording a tape per step, then reverses the segment step
+by step.  A replayed step records exactly the tape of the forward step (the
+same kernels on the same state, with the forward's iteration counts), so
+the gradients do not depend on the budget; the budget only trades HBM for
+the replay work.  A tape holds the sorted positions, masses, rest
 densities, pids and cell table, vel_adv and kappa of every Jacobi iteration,
 and the advection's velocity and normals.  Neighbour lists, densities and
 alpha factors are not stored: `tsph_relist` recomputes them bit-identically
@@ -223,25 +229,41 @@ class DiffRunner:
 
 class _Rollout(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, v0, mu, gamma, runner, steps, every):
+    def forward(ctx, v0, mu, gamma, runner, steps, every, budget):
         runner.set_params(mu.item(), gamma.item())
         runner.surface_tension_grad = bool(gamma.requires_grad) or gamma.item() != 0.0
         runner.set_velocity_ref(v0.detach())
         runner.step_index = 0
         runner.iter_log = []
         ckpts = []
-        last = ((steps - 1) // every) * every  # first step of the last segment
-        last_tapes = []
+        nseg = (steps + every - 1) // every
+        kept = {}  # segment -> its tapes, for the last segment and those within the budget
+        held = []  # (segment, bytes) of the kept earlier segments, oldest first
+        used = 0
         for t in range(steps):
+            s = t // every
             if t % every == 0:
                 ckpts.append(runner.checkpoint())
+                kept[s] = []
             # the last segment is reversed first: record its tapes now instead
-            # of replaying it in the backward pass
-            tp = runner.step(tape=t >= last)
-            if tp is not None:
-                last_tapes.append(tp)
+            # of replaying it in the backward pass; earlier ones while they fit
+            tp = runner.step(tape=s in kept)
+            if tp is None:
+                continue
+            kept[s].append(tp)
+            if s == nseg - 1:
+                continue
+            if not held or held[-1][0] != s:
+                held.append([s, 0])
+            b = tp.nbytes()
+            held[-1][1] += b
+            used += b
+            while held and used > budget:
+                s_old, b_old = held.pop(0)
+                used -= b_old
+                del kept[s_old]
         ctx.runner, ctx.ckpts, ctx.steps, ctx.every = runner, ckpts, steps, every
-        ctx.last_tapes = last_tapes
+        ctx.kept = kept
         ctx.mu, ctx.gamma = float(mu.item()), float(gamma.item())
         ctx.v0_dtype = v0.dtype
         return runner.positions_ref(), runner.velocities_ref()
@@ -267,8 +289,8 @@ class _Rollout(torch.autograd.Function):
         for s in reversed(range(len(ctx.ckpts))):
             t0 = s * ctx.every
             t1 = min(t0 + ctx.every, ctx.steps)
-            if s == len(ctx.ckpts) - 1:
-                tapes, ctx.last_tapes = ctx.last_tapes, None
+            if s in ctx.kept:
+                tapes = ctx.kept.pop(s)
             else:
                 runner.restore(ctx.ckpts[s])
                 tapes = [runner.step(tape=True) for _ in range(t0, t1)]
@@ -280,17 +302,20 @@ class _Rollout(torch.autograd.Function):
         g = runner.grad.to(torch.float64)
         gmu = g[0].to(ctx.v0_dtype)
         ggam = g[1].to(ctx.v0_dtype)
-        return vbar[:, :d].to(ctx.v0_dtype), gmu, ggam, None, None, None
+        return vbar[:, :d].to(ctx.v0_dtype), gmu, ggam, None, None, None, None
 
 
-def rollout(runner, v0, mu, gamma, steps, every=20):
+def rollout(runner, v0, mu, gamma, steps, every=20, tape_budget=0):
     """Differentiable rollout of `steps` fused DFSPH steps from the runner's
     current state with initial fluid velocity v0 ([n, dim], reference
     order), dynamic viscosity mu and surface-tension coefficient gamma (0-d
     tensors).  Returns final (positions, velocities) in reference order.
-    The state is checkpointed every `every` steps."""
+    The state is checkpointed every `every` steps; the tapes of the last
+    segment, and of the latest earlier segments whose tapes fit in
+    `tape_budget` bytes of HBM, are kept from the forward pass instead of
+    being replayed in the backward pass."""
     if not torch.is_tensor(mu):
         mu = torch.tensor(float(mu), dtype=v0.dtype, device=v0.device)
     if not torch.is_tensor(gamma):
         gamma = torch.tensor(float(gamma), dtype=v0.dtype, device=v0.device)
-    return _Rollout.apply(v0, mu, gamma, runner, int(steps), max(int(every), 1))
+    return _Rollout.apply(v0, mu, gamma, runner, int(steps), max(int(every), 1), max(int(tape_budget), 0))

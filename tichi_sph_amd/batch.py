@@ -83,7 +83,7 @@ class SceneBatch:
         return ctypes.c_void_p(self.table.data_ptr())
 
     def _call(self, name, fn, *args):
-        if self.timer is None:
+        if self.timer is None or not self.timer.wants(name):
             _lib.check(fn(*args), name)
             return
         e0, e1 = self.timer.pair()

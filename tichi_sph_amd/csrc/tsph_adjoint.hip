@@ -264,20 +264,27 @@ __global__ void __launch_bounds__(kBlock)
       const T rhoi = dens[i], r0i = rdens[i], mi = fmass[i];
       int c[3];
       node_encode(k, p.x, p.y, p.z, c);
-      walk_cells(k, fstart, c, [&](int j) {
-        V4<T> q = fpos[j];
+      struct Cand {
+        V4<T> q, nj, uj;
+        T vfj, rhoj, r0j, mj;
+      };
+      auto load = [&](int j) {
+        return Cand{fpos[j], tnormal[j], vbar[j], vf[(long)j * k.P], dens[j], rdens[j], fmass[j]};
+      };
+      walk_cells_prefetch<kStU>(k, fstart, c, load, [&](int, const Cand& d) {
+        const V4<T>& q = d.q;
         T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
         T r2 = dx * dx + dy * dy + dz * dz;
-        if (!(r2 > T(0)) || !(vf[(long)j * k.P] > T(0.99))) return;
+        if (!(r2 > T(0)) || !(d.vfj > T(0.99))) return;
         T r = s_sqrt(r2);
         T C, Cp;
         coh_d1(k, r, &C, &Cp);
         T cr = C / r;
         T cq = (Cp - cr) / r2;
-        V4<T> nj = tnormal[j];
-        V4<T> uj = vbar[j];
+        const V4<T>& nj = d.nj;
+        const V4<T>& uj = d.uj;
         T alj_x = dt * uj.x, alj_y = dt * uj.y, alj_z = dt * uj.z;
-        T rhoj = dens[j], r0j = rdens[j], mj = fmass[j];
+        T rhoj = d.rhoj, r0j = d.r0j, mj = d.mj;
         T den = rhoi + rhoj;
         T fij = T(2) * r0i / den, fji = T(2) * r0j / den;
         T dnx = ni.x - nj.x, dny = ni.y - nj.y, dnz = ni.z - nj.z;

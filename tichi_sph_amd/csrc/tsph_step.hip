@@ -482,14 +482,21 @@ __device__ __forceinline__ void k_surface_tension_body(Blk blk, KP<T> k, int n, 
     T psi_i = psi_rho[i];
     int c[3];
     node_encode(k, p.x, p.y, p.z, c);
-    walk_cells(k, fstart, c, [&](int j) {
-      V4<T> q = fpos[j];
+    struct Cand {
+      V4<T> q, nj;
+      T mj, psij, vfj;
+    };
+    auto load = [&](int j) {
+      return Cand{fpos[j], normal[j], fmass[j], psi_rho[j], vf[(long)j * k.P]};
+    };
+    walk_cells_prefetch<kStU>(k, fstart, c, load, [&](int, const Cand& d) {
+      const V4<T>& q = d.q;
       T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
       T r = s_sqrt(s_r2(dx, dy, dz));
-      if (r > T(0) && vf[(long)j * k.P] > T(0.99)) {
-        T cs = -k.gamma * fmass[j] * s_C(k, r);
-        V4<T> nj = normal[j];
-        T f = T(2) * rp / (psi_i + psi_rho[j]);
+      if (r > T(0) && d.vfj > T(0.99)) {
+        T cs = -k.gamma * d.mj * s_C(k, r);
+        const V4<T>& nj = d.nj;
+        T f = T(2) * rp / (psi_i + d.psij);
         a.x = a.x + f * (cs * dx / r + k.gamma * (ni.x - nj.x));
         a.y = a.y + f * (cs * dy / r + k.gamma * (ni.y - nj.y));
         a.z = a.z + f * (cs * dz / r + k.gamma * (ni.z - nj.z));

@@ -41,27 +41,35 @@ __device__ __forceinline__ void walk_cells(const KP<T>& k, const int* __restrict
   });
 }
 
-// Candidate walk of pass A: the candidates of a run are gathered U at a
-// time, all U position loads in flight together, then f(j, q) runs for each
-// in walk order.  The walk is bound by the latency of these gathers (each
-// trip of the one-at-a-time loop waited a full round trip), not by their
-// number.
+// Candidate walk with prefetch: the records load(j) of a run's candidates are
+// gathered U at a time, all U in flight together, then f(j, record) runs for
+// each in walk order.  Cell walks are bound by the latency of these gathers
+// (a one-candidate-at-a-time loop waits a full round trip per candidate),
+// not by their number.
+template <int U, class T, class Load, class F>
+__device__ __forceinline__ void walk_cells_prefetch(const KP<T>& k, const int* __restrict__ cstart,
+                                                    const int* c, Load&& load, F&& f) {
+  walk_runs(k, cstart, c, [&](int j0, int j1) {
+    for (int j = j0; j < j1; j += U) {
+      decltype(load(j0)) rec[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) rec[u] = load(j + u < j1 ? j + u : j);
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (j + u < j1) f(j + u, rec[u]);
+    }
+  });
+}
+
+// The pass A form: the prefetched record is the candidate's position.
 template <int U, class T, class F>
 __device__ __forceinline__ void walk_cells_batched(const KP<T>& k, const int* __restrict__ cstart,
                                                    const int* c, const V4<T>* __restrict__ pos,
                                                    F&& f) {
-  walk_runs(k, cstart, c, [&](int j0, int j1) {
-    for (int j = j0; j < j1; j += U) {
-      V4<T> q[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) q[u] = pos[j + u < j1 ? j + u : j];
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-        if (j + u < j1) f(j + u, q[u]);
-    }
-  });
+  walk_cells_prefetch<U>(k, cstart, c, [&](int j) { return pos[j]; }, f);
 }
 constexpr int kPassAU = 8;
+constexpr int kStU = 4;  // prefetch depth of the surface-tension cell walks
 
 template <class T>
 __device__ __forceinline__ T rest_psi_of(const KP<T>& k, const T* rdens, int i) {

@@ -46,13 +46,18 @@ class StepStats:
 class EventTimer:
     """Per-call HIP-event timing: each timed library call records a start and
     an end event on the launch stream.  Events come from a pool created up
-    front (`reserve`), so timing adds no event creation inside a timed loop."""
+    front (`reserve`), so timing adds no event creation inside a timed loop.
+    With `only` (a set of call names) the other calls are not timed."""
 
-    def __init__(self, reserve=0):
+    def __init__(self, reserve=0, only=None):
         self.pool = []
         self.used = 0
         self.records = []  # (name, start event, end event)
+        self.only = None if only is None else set(only)
         self.reserve(reserve)
+
+    def wants(self, name):
+        return self.only is None or name in self.only
 
     def reserve(self, n):
         while len(self.pool) < n:
@@ -175,7 +180,7 @@ class Simulation:
     def _call(self, name, fn, *args):
         self._require_gpu(name)
         self.calls += 1
-        if self.timer is None:
+        if self.timer is None or not self.timer.wants(name):
             _lib.check(fn(*args), name)
             return
         e0, e1 = self.timer.pair()
