@@ -143,9 +143,13 @@ __global__ void __launch_bounds__(kBlock)
                 const int* __restrict__ nbr_b, const int* __restrict__ nbr_nf,
                 const int* __restrict__ nbr_nb, V4<T>* __restrict__ tvel, T* __restrict__ rhobar,
                 T* __restrict__ alphabar, V4<T>* __restrict__ xbar, const TsphControl* ctrl) {
-  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (i >= n) return;
+  const int base = xcd_block(blockIdx.x, gridDim.x) * kBlock;
+  const int cl = coop_lane<kCoop>();
   const T dt = (T)ctrl->dt;
+#pragma unroll 1
+  for (int round = 0; round < kCoop; ++round) {
+  const int i = coop_particle<kCoop>(base, round);
+  if (i >= n) break;
   const V4<T> p = fpos[i];
   const T Xi = p.w;
   const V4<T> u = vbar[i];
@@ -164,11 +168,16 @@ __global__ void __launch_bounds__(kBlock)
     T c = ki * q.w + uj.w * Xi;
     add_J(ax, ay, az, e, dx, dy, dz, c * mx, c * my, c * mz);
   };
-  sweep_lists<T, Rec2<T>>(
-      k, cap, i, p, fstart, bstart, fpos, bpos, nbr_f, nbr_b, nbr_nf[i], nbr_nb[i],
+  sweep_lists_coop<kCoop, T, Rec2<T>>(
+      k, cap, i, cl, p, fstart, bstart, fpos, bpos, nbr_f, nbr_b, nbr_nf[i], nbr_nb[i],
       [&](int j) { return Rec2<T>{fpos[j], vbar[j]}; },
       [&](int j) { return Rec2<T>{bpos[j], v4(T(0), T(0), T(0), T(0))}; },
       [&](const Rec2<T>& d) { term(d, true); }, [&](const Rec2<T>& d) { term(d, false); });
+  kb = group_sum<kCoop>(kb);
+  ax = group_sum<kCoop>(ax);
+  ay = group_sum<kCoop>(ay);
+  az = group_sum<kCoop>(az);
+  if (cl != 0) continue;
   const T s = -(T(1) / dt);
   kb *= s;
   const T al = alpha[i];
@@ -178,6 +187,7 @@ __global__ void __launch_bounds__(kBlock)
   alphabar[i] += -kb * ki / al;
   V4<T> xb = xbar[i];
   xbar[i] = v4(xb.x + s * ax, xb.y + s * ay, xb.z + s * az);
+  }
 }
 
 // Jacobi iteration reverse, half 2: IPPE_adv_psi (sph.py:202-216)
@@ -196,9 +206,13 @@ __global__ void __launch_bounds__(kBlock)
               const int* __restrict__ nbr_f, const int* __restrict__ nbr_b,
               const int* __restrict__ nbr_nf, const int* __restrict__ nbr_nb,
               V4<T>* __restrict__ vbar_out, V4<T>* __restrict__ xbar, const TsphControl* ctrl) {
-  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (i >= n) return;
+  const int base = xcd_block(blockIdx.x, gridDim.x) * kBlock;
+  const int cl = coop_lane<kCoop>();
   const T dt = (T)ctrl->dt;
+#pragma unroll 1
+  for (int round = 0; round < kCoop; ++round) {
+  const int i = coop_particle<kCoop>(base, round);
+  if (i >= n) break;
   const V4<T> p = fpos[i];
   const T Xi = p.w;
   const V4<T> vi = tvel[i];  // w = psi_bar of this particle
@@ -216,15 +230,23 @@ __global__ void __launch_bounds__(kBlock)
     vz += cg * dz;
     add_J(ax, ay, az, e, dx, dy, dz, c * (vi.x - vj.x), c * (vi.y - vj.y), c * (vi.z - vj.z));
   };
-  sweep_lists<T, Rec2<T>>(
-      k, cap, i, p, fstart, bstart, fpos, bpos, nbr_f, nbr_b, nbr_nf[i], nbr_nb[i],
+  sweep_lists_coop<kCoop, T, Rec2<T>>(
+      k, cap, i, cl, p, fstart, bstart, fpos, bpos, nbr_f, nbr_b, nbr_nf[i], nbr_nb[i],
       [&](int j) { return Rec2<T>{fpos[j], tvel[j]}; },
       [&](int j) { return Rec2<T>{bpos[j], v4(T(0), T(0), T(0), T(0))}; }, term, term);
+  vx = group_sum<kCoop>(vx);
+  vy = group_sum<kCoop>(vy);
+  vz = group_sum<kCoop>(vz);
+  ax = group_sum<kCoop>(ax);
+  ay = group_sum<kCoop>(ay);
+  az = group_sum<kCoop>(az);
+  if (cl != 0) continue;
   const V4<T> u = vbar[i];
   vbar_out[i] = v4(u.x + dt * vx, u.y + dt * vy, u.z + dt * vz,
                    kappa_next ? kappa_next[i] : T(0));
   V4<T> xb = xbar[i];
   xbar[i] = v4(xb.x + dt * ax, xb.y + dt * ay, xb.z + dt * az);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -346,9 +368,13 @@ __global__ void __launch_bounds__(kBlock)
                     V4<T>* __restrict__ xbar, double* __restrict__ partial,
                     const TsphControl* ctrl) {
   __shared__ double lds[kWaves];
-  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int base = xcd_block(blockIdx.x, gridDim.x) * kBlock;
+  const int cl = coop_lane<kCoop>();
   double msum = 0;
-  if (i < n) {
+#pragma unroll 1
+  for (int round = 0; round < kCoop; ++round) {
+    const int i = coop_particle<kCoop>(base, round);
+    if (i >= n) break;
     const T dt = (T)ctrl->dt;
     const V4<T> p = fpos[i];
     const V4<T> vi = tvel[i];
@@ -395,8 +421,8 @@ __global__ void __launch_bounds__(kBlock)
       xby += -k.h * (e.g * cy + s * dy);
       xbz += -k.h * (e.g * cz + s * dz);
     };
-    sweep_lists<T, Rec4<T>>(
-        k, cap, i, p, fstart, bstart, fpos, bpos, nbr_f, nullptr, nbr_nf[i], 0,
+    sweep_lists_coop<kCoop, T, Rec4<T>>(
+        k, cap, i, cl, p, fstart, bstart, fpos, bpos, nbr_f, nullptr, nbr_nf[i], 0,
         [&](int j) {
           V4<T> v = tvel[j];
           v.w = vol[j];
@@ -405,11 +431,20 @@ __global__ void __launch_bounds__(kBlock)
           return Rec4<T>{fpos[j], v, a, nbar ? nbar[j] : V4<T>{}};
         },
         [&](int j) { return Rec4<T>{}; }, term, [&](const Rec4<T>&) {});
+    vbx = group_sum<kCoop>(vbx);
+    vby = group_sum<kCoop>(vby);
+    vbz = group_sum<kCoop>(vbz);
+    xbx = group_sum<kCoop>(xbx);
+    xby = group_sum<kCoop>(xby);
+    xbz = group_sum<kCoop>(xbz);
+    Vb = group_sum<kCoop>(Vb);
+    ms = group_sum<kCoop>(ms);
+    if (cl != 0) continue;
     vbar_out[i] = v4(ui.x + vbx, ui.y + vby, ui.z + vbz, kappa_next ? kappa_next[i] : T(0));
     rhobar[i] += -Vb * Vi / dens[i];
     V4<T> xb = xbar[i];
     xbar[i] = v4(xb.x + xbx, xb.y + xby, xb.z + xbz);
-    msum = (double)ms;
+    msum += (double)ms;
   }
   double s = block_sum(msum, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
@@ -455,8 +490,12 @@ __global__ void __launch_bounds__(kBlock)
                  const int* __restrict__ bstart, const int* __restrict__ nbr_f,
                  const int* __restrict__ nbr_b, const int* __restrict__ nbr_nf,
                  const int* __restrict__ nbr_nb, V4<T>* __restrict__ xbar) {
-  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (i >= n) return;
+  const int base = xcd_block(blockIdx.x, gridDim.x) * kBlock;
+  const int cl = coop_lane<kCoop>();
+#pragma unroll 1
+  for (int round = 0; round < kCoop; ++round) {
+  const int i = coop_particle<kCoop>(base, round);
+  if (i >= n) break;
   const V4<T> p = fpos[i];
   const T Xi = p.w, mi = fmass[i];
   const T rbi = rhobar[i];
@@ -465,8 +504,8 @@ __global__ void __launch_bounds__(kBlock)
   T ax = 0, ay = 0, az = 0;
   // record: a = position (w = X_j), b = a1bar_j (w = a2bar_j),
   // c = (rho_bar_j, m_j, is_fluid)
-  sweep_lists<T, Rec4<T>>(
-      k, cap, i, p, fstart, bstart, fpos, bpos, nbr_f, nbr_b, nbr_nf[i], nbr_nb[i],
+  sweep_lists_coop<kCoop, T, Rec4<T>>(
+      k, cap, i, cl, p, fstart, bstart, fpos, bpos, nbr_f, nbr_b, nbr_nf[i], nbr_nb[i],
       [&](int j) {
         V4<T> q = fpos[j];
         V4<T> extra = v4(rhobar[j], q.w, T(1), T(0));  // DFSPH: m_j = X_j
@@ -503,8 +542,13 @@ __global__ void __launch_bounds__(kBlock)
         az += cd * dz;
         add_J(ax, ay, az, e, dx, dy, dz, q.w * abi.x, q.w * abi.y, q.w * abi.z);
       });
+  ax = group_sum<kCoop>(ax);
+  ay = group_sum<kCoop>(ay);
+  az = group_sum<kCoop>(az);
+  if (cl != 0) continue;
   V4<T> xb = xbar[i];
   xbar[i] = v4(xb.x + ax, xb.y + ay, xb.z + az);
+  }
 }
 
 // ---------------------------------------------------------------------------

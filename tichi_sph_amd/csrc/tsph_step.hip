@@ -259,14 +259,17 @@ __device__ __forceinline__ void k_solver_psi_body(Blk blk, KP<T> k, int n, int c
                  double* __restrict__ partial, TsphControl* ctrl) {
   __shared__ double lds[kWaves];
   if (!ctrl->run_u[kk]) return;
-  const int i = xcd_block(blk.b, blk.nb) * kBlock + threadIdx.x;
+  const int base = xcd_block(blk.b, blk.nb) * kBlock;
+  const int c = coop_lane<kCoop>();
+  const T dt = (T)ctrl->dt;
   double err = 0;
-  if (i < n) {
-    const T dt = (T)ctrl->dt;
+#pragma unroll 1
+  for (int round = 0; round < kCoop; ++round) {
+    const int i = coop_particle<kCoop>(base, round);
+    if (i >= n) break;
     const V4<T> p = fpos[i];
     const V4<T> va = vadv[i];
-    const T rp = rest_psi_of(k, rdens, i);
-    T psi = psi_rho[i] - rp;  // a single rounding
+    T sum = 0;  // this lane's share of sum_j X_j (v_i - v_j) . grad W_ij
     auto term = [&](const Rec2<T>& d) {
       const V4<T>& q = d.a;
       const V4<T>& vj = d.b;
@@ -274,13 +277,17 @@ __device__ __forceinline__ void k_solver_psi_body(Blk blk, KP<T> k, int n, int c
       T r2 = dx * dx + dy * dy + dz * dz, r;
       T g = f_grad(k, r2, &r);
       T dot = (dx * (va.x - vj.x) + dy * (va.y - vj.y) + dz * (va.z - vj.z)) * g;
-      psi += dot * q.w * dt;
+      sum += dot * q.w;
     };
-    sweep_lists<T, Rec2<T>>(
-        k, cap, i, p, fstart, bstart, fpos, bpos, nbr_f, nbr_b, nbr_nf[i], nbr_nb[i],
+    sweep_lists_coop<kCoop, T, Rec2<T>>(
+        k, cap, i, c, p, fstart, bstart, fpos, bpos, nbr_f, nbr_b, nbr_nf[i], nbr_nb[i],
         [&](int j) { return Rec2<T>{fpos[j], vadv[j]}; },
         // boundary vel_adv is never written: 0
         [&](int j) { return Rec2<T>{bpos[j], V4<T>{T(0), T(0), T(0), T(0)}}; }, term, term);
+    sum = group_sum<kCoop>(sum);
+    if (c != 0) continue;
+    const T rp = rest_psi_of(k, rdens, i);
+    T psi = (psi_rho[i] - rp) + sum * dt;
     if (psi < T(0)) psi = T(0);
     psi_adv[i] = psi;
     const T kap = psi / alpha[i];
@@ -290,7 +297,7 @@ __device__ __forceinline__ void k_solver_psi_body(Blk blk, KP<T> k, int n, int c
       tape_v[i] = va;
       tape_k[i] = kap;
     }
-    err = (double)(psi / rp);
+    err += (double)(psi / rp);
   }
   double s = block_sum(err, lds);
   if (threadIdx.x == 0) partial[blk.b] = s;
@@ -358,34 +365,43 @@ __device__ __forceinline__ void k_solver_update_body(Blk blk, KP<T> k, int n, in
       }
     }
   }
-  const int i = xcd_block(blk.b, blk.nb) * kBlock + threadIdx.x;
-  if (i >= n) return;
+  const int base = xcd_block(blk.b, blk.nb) * kBlock;
+  const int cl = coop_lane<kCoopU>();
   const T dt = (T)ctrl->dt;
-  const V4<T> p = fpos[i];
-  V4<T> va = vadv[i];
-  const T ki = kappa[i];
-  const T Xi = p.w;
-  const T c = -(T(1) / dt) / fmass[i];
-  const V4<T> A = alpha_1[i];
-  const T cA = c * ki;
-  va.x += cA * A.x;
-  va.y += cA * A.y;
-  va.z += cA * A.z;
-  const T cX = c * Xi;
-  auto term = [&](const V4<T>& q) {  // q = (x_j, kappa_j)
-    T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-    T r2 = dx * dx + dy * dy + dz * dz, r;
-    T g = f_grad(k, r2, &r);
-    T s = cX * q.w * g;
-    va.x += s * dx;
-    va.y += s * dy;
-    va.z += s * dz;
-  };
-  sweep_lists<T, V4<T>>(
-      k, cap, i, p, fstart, bstart, fpos, bpos, nbr_f, nullptr, nbr_nf[i], 0,
-      [&](int j) { return krec[j]; }, [&](int) { return V4<T>{}; }, term,
-      [&](const V4<T>&) {});
-  vadv[i] = va;
+#pragma unroll 1
+  for (int round = 0; round < kCoopU; ++round) {
+    const int i = coop_particle<kCoopU>(base, round);
+    if (i >= n) break;
+    const V4<T> p = fpos[i];
+    const T Xi = p.w;
+    T sx = 0, sy = 0, sz = 0;  // this lane's share of sum_j kappa_j grad W_ij
+    auto term = [&](const V4<T>& q) {  // q = (x_j, kappa_j)
+      T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+      T r2 = dx * dx + dy * dy + dz * dz, r;
+      T g = f_grad(k, r2, &r);
+      T s = q.w * g;
+      sx += s * dx;
+      sy += s * dy;
+      sz += s * dz;
+    };
+    sweep_lists_coop<kCoopU, T, V4<T>>(
+        k, cap, i, cl, p, fstart, bstart, fpos, bpos, nbr_f, nullptr, nbr_nf[i], 0,
+        [&](int j) { return krec[j]; }, [&](int) { return V4<T>{}; }, term,
+        [&](const V4<T>&) {});
+    sx = group_sum<kCoopU>(sx);
+    sy = group_sum<kCoopU>(sy);
+    sz = group_sum<kCoopU>(sz);
+    if (cl != 0) continue;
+    V4<T> va = vadv[i];
+    const T ki = kappa[i];
+    const T c = -(T(1) / dt) / fmass[i];
+    const V4<T> A = alpha_1[i];
+    const T cA = c * ki, cX = c * Xi;
+    va.x += cA * A.x + cX * sx;
+    va.y += cA * A.y + cX * sy;
+    va.z += cA * A.z + cX * sz;
+    vadv[i] = va;
+  }
 }
 
 template <class T>
@@ -414,38 +430,49 @@ __device__ __forceinline__ void k_advect_body(Blk blk, KP<T> k, int n, int cap, 
              const int* __restrict__ nbr_f, const int* __restrict__ nbr_nf,
              V4<T>* __restrict__ vadv, V4<T>* __restrict__ vel, V4<T>* __restrict__ acce_adv,
              V4<T>* __restrict__ normal) {
-  const int i = xcd_block(blk.b, blk.nb) * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const V4<T> p = fpos[i];
-  const V4<T> vi = vadv[i];
-  const T fvis = k.two_d2 * k.mu / rdens[i];
-  T ax = T(0) + k.g[0], ay = T(0) + k.g[1], az = T(0) + k.g[2];
-  T nx = 0, ny = 0, nz = 0;
-  auto term = [&](const Rec2<T>& d) {
-    const V4<T>& q = d.a;
-    const V4<T>& vj = d.b;
-    T Vj = vj.w;  // V_j = X_j / sph_psi_j rides in vel_adv's 4th lane (pass A)
-    T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-    T r2 = dx * dx + dy * dy + dz * dz, r;
-    T g = f_grad(k, r2, &r);
-    T Ax = (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz;
-    T s = fvis * Vj * g * Ax / (k.eps + r2);  // W_lap, sph_util.py:40-43
-    ax += s * dx;
-    ay += s * dy;
-    az += s * dz;
-    T t = -Vj * g;  // normal, sph.py:149
-    nx += t * dx;
-    ny += t * dy;
-    nz += t * dz;
-  };
-  // viscosity and normals are (f,f) only: the boundary list is not visited
-  sweep_lists<T, Rec2<T>>(
-      k, cap, i, p, fstart, bstart, fpos, bpos, nbr_f, nullptr, nbr_nf[i], 0,
-      [&](int j) { return Rec2<T>{fpos[j], vadv[j]}; },
-      [&](int j) { return Rec2<T>{}; }, term, [&](const Rec2<T>&) {});
-  vel[i] = V4<T>{vi.x, vi.y, vi.z, T(0)};  // SPH_vel_adv_2_vel
-  acce_adv[i] = V4<T>{ax, ay, az, T(0)};
-  normal[i] = V4<T>{nx * k.h, ny * k.h, nz * k.h, T(0)};
+  const int base = xcd_block(blk.b, blk.nb) * kBlock;
+  const int c = coop_lane<kCoop>();
+#pragma unroll 1
+  for (int round = 0; round < kCoop; ++round) {
+    const int i = coop_particle<kCoop>(base, round);
+    if (i >= n) break;
+    const V4<T> p = fpos[i];
+    const V4<T> vi = vadv[i];
+    const T fvis = k.two_d2 * k.mu / rdens[i];
+    T ax = 0, ay = 0, az = 0, nx = 0, ny = 0, nz = 0;  // this lane's share
+    auto term = [&](const Rec2<T>& d) {
+      const V4<T>& q = d.a;
+      const V4<T>& vj = d.b;
+      T Vj = vj.w;  // V_j = X_j / sph_psi_j rides in vel_adv's 4th lane (pass A)
+      T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+      T r2 = dx * dx + dy * dy + dz * dz, r;
+      T g = f_grad(k, r2, &r);
+      T Ax = (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz;
+      T s = fvis * Vj * g * Ax / (k.eps + r2);  // W_lap, sph_util.py:40-43
+      ax += s * dx;
+      ay += s * dy;
+      az += s * dz;
+      T t = -Vj * g;  // normal, sph.py:149
+      nx += t * dx;
+      ny += t * dy;
+      nz += t * dz;
+    };
+    // viscosity and normals are (f,f) only: the boundary list is not visited
+    sweep_lists_coop<kCoop, T, Rec2<T>>(
+        k, cap, i, c, p, fstart, bstart, fpos, bpos, nbr_f, nullptr, nbr_nf[i], 0,
+        [&](int j) { return Rec2<T>{fpos[j], vadv[j]}; },
+        [&](int j) { return Rec2<T>{}; }, term, [&](const Rec2<T>&) {});
+    ax = group_sum<kCoop>(ax);
+    ay = group_sum<kCoop>(ay);
+    az = group_sum<kCoop>(az);
+    nx = group_sum<kCoop>(nx);
+    ny = group_sum<kCoop>(ny);
+    nz = group_sum<kCoop>(nz);
+    if (c != 0) continue;
+    vel[i] = V4<T>{vi.x, vi.y, vi.z, T(0)};  // SPH_vel_adv_2_vel
+    acce_adv[i] = V4<T>{k.g[0] + ax, k.g[1] + ay, k.g[2] + az, T(0)};
+    normal[i] = V4<T>{nx * k.h, ny * k.h, nz * k.h, T(0)};
+  }
 }
 
 template <class T>

@@ -183,4 +183,99 @@ struct Rec2 {
   V4<T> a, b;
 };
 
+// ---------------------------------------------------------------------------
+// Cooperative list sweeps.  A group of kCoop adjacent lanes shares one
+// particle; lane c of the group takes list slots c, c + kCoop, c + 2 kCoop,
+// ...  At each step the group's lanes gather kCoop consecutive neighbours of
+// the same particle, which lie in one or two cells, and the 64 lanes of a
+// wave gather from about half as many cache sectors as with one lane per
+// particle (whose lanes drift apart through their lists).  The sweeps are
+// bound by the number of L1 sectors their gathers touch, not by the number
+// of gathers.  The group's partial sums are combined by a butterfly, which
+// leaves the same value in every lane of the group.
+// ---------------------------------------------------------------------------
+constexpr int kCoop = 4;    // lanes per particle of the gather-heavy sweeps
+constexpr int kCoopU = 2;   // lanes per particle of the update sweep (one gather per pair)
+constexpr int kCoopNb = 4;  // gathers in flight per lane
+
+// A block covers its kBlock particles in C rounds of kBlock / C particles.
+template <int C>
+__device__ __forceinline__ int coop_lane() {
+  return threadIdx.x & (C - 1);
+}
+template <int C>
+__device__ __forceinline__ int coop_particle(int base, int round) {
+  return base + round * (kBlock / C) + (int)(threadIdx.x / C);
+}
+
+template <int C, class T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+  for (int o = 1; o < C; o <<= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// One list, lane c's share, software-pipelined like sweep_one_list.
+template <int C, class Rec, class Load, class Pair>
+__device__ __forceinline__ void sweep_one_list_coop(int cap, int i, int c,
+                                                    const int* __restrict__ list, int n,
+                                                    Load&& load, Pair&& pair) {
+  int j[kCoopNb];
+#pragma unroll
+  for (int u = 0; u < kCoopNb; ++u) {
+    const int s = c + C * u;
+    j[u] = s < n ? list[(long)s * cap + i] : -1;
+  }
+  for (int s0 = c; s0 < n; s0 += C * kCoopNb) {
+    Rec d[kCoopNb];
+#pragma unroll
+    for (int u = 0; u < kCoopNb; ++u)
+      if (j[u] >= 0) d[u] = load(j[u]);
+    int jn[kCoopNb];
+#pragma unroll
+    for (int u = 0; u < kCoopNb; ++u) {
+      const int s = s0 + C * (kCoopNb + u);
+      jn[u] = s < n ? list[(long)s * cap + i] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < kCoopNb; ++u)
+      if (j[u] >= 0) pair(d[u]);
+#pragma unroll
+    for (int u = 0; u < kCoopNb; ++u) j[u] = jn[u];
+  }
+}
+
+// sweep_lists for lane c of a particle's group.  A particle whose list
+// overflowed is swept by lane 0 alone, walking the cells.
+template <int C, class T, class Rec, class LoadF, class LoadB, class PairF, class PairB>
+__device__ __forceinline__ void sweep_lists_coop(const KP<T>& k, int cap, int i, int c, V4<T> p,
+                                                 const int* __restrict__ fstart,
+                                                 const int* __restrict__ bstart,
+                                                 const V4<T>* __restrict__ fpos,
+                                                 const V4<T>* __restrict__ bpos,
+                                                 const int* __restrict__ nbr_f,
+                                                 const int* __restrict__ nbr_b, int nf, int nbb,
+                                                 LoadF&& load_f, LoadB&& load_b, PairF&& pair_f,
+                                                 PairB&& pair_b) {
+  if (nf <= k.list_cap_f && nbb <= k.list_cap_b) {
+    sweep_one_list_coop<C, Rec>(cap, i, c, nbr_f, nf, load_f, pair_f);
+    if (nbb > 0) sweep_one_list_coop<C, Rec>(cap, i, c, nbr_b, nbb, load_b, pair_b);
+    return;
+  }
+  if (c != 0) return;
+  int cc[3];
+  node_encode(k, p.x, p.y, p.z, cc);
+  walk_cells(k, fstart, cc, [&](int j) {
+    V4<T> q = fpos[j];
+    T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
+    if (r2 > T(0) && r2 <= k.r2max) pair_f(load_f(j));
+  });
+  if (nbr_b == nullptr) return;
+  walk_cells(k, bstart, cc, [&](int j) {
+    V4<T> q = bpos[j];
+    T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
+    if (r2 > T(0) && r2 <= k.r2max) pair_b(load_b(j));
+  });
+}
+
 }  // namespace tsph
