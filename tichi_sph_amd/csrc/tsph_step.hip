@@ -218,6 +218,31 @@ __device__ __forceinline__ void step_reset_body(TsphControl* ctrl) {
 }
 __global__ void k_step_reset(TsphControl* ctrl) { step_reset_body(ctrl); }
 
+// Pass A of both objects in one launch: blocks [0, nbf) run the fluid body,
+// the rest the boundary body.  The two halves are independent (each writes
+// only its own object's fields), so the boundary blocks fill the tail of the
+// fluid's grid instead of waiting for it.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_pass_a_both(KP<T> k, TsphObject f, TsphObject b, int nbf, TsphControl* ctrl) {
+  if ((int)blockIdx.x < nbf) {
+    k_pass_a_fluid_body<T>(Blk{(int)blockIdx.x, nbf}, k, f.n, f.capacity, (const V4<T>*)f.pos,
+                           (const T*)f.mass, (const T*)f.rest_volume, f.cell_start, b.n,
+                           (const V4<T>*)b.pos, (const T*)b.mass, (const T*)b.rest_volume,
+                           b.cell_start, (T*)f.W, (T*)f.sph_compression, (T*)f.sph_density,
+                           (V4<T>*)f.alpha_1, (T*)f.alpha_2, (T*)f.alpha, (T*)f.vol,
+                           (const V4<T>*)f.vel, (V4<T>*)f.vel_adv, f.nbr_f, f.nbr_b, f.nbr_nf,
+                           f.nbr_nb, f.flag, ctrl);
+  } else {
+    k_pass_a_bound_body<T>(Blk{(int)blockIdx.x - nbf, (int)gridDim.x - nbf}, k, b.n,
+                           (const V4<T>*)b.pos, (const T*)b.mass, (const T*)b.rest_volume,
+                           b.cell_start, (const V4<T>*)f.pos, (const T*)f.mass,
+                           (const T*)f.rest_volume, f.cell_start, (T*)b.W,
+                           (T*)b.sph_compression, (T*)b.sph_density, (V4<T>*)b.alpha_1,
+                           (T*)b.alpha_2, (T*)b.alpha, (T*)b.vol);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Solver stop state (main.py:201-213 / 222-234).
 // ---------------------------------------------------------------------------
@@ -743,6 +768,13 @@ static int pass_a(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphContr
                   bool with_bound, hipStream_t st) {
   KP<T> k = make_kp<T>(prm);
   hipLaunchKernelGGL(k_step_reset, dim3(1), dim3(64), 0, st, ctrl);
+  if (with_bound && f->n > 0 && b->n > 0) {
+    const int nbf = blocks_for(f->n);
+    hipLaunchKernelGGL(k_pass_a_both<T>, dim3(nbf + blocks_for(b->n)), dim3(kBlock), 0, st, k, *f,
+                       *b, nbf, ctrl);
+    TSPH_LAUNCH_CHECK("density_alpha");
+    return 0;
+  }
   if (f->n > 0)
     hipLaunchKernelGGL(k_pass_a_fluid<T>, dim3(blocks_for(f->n)), dim3(kBlock), 0, st, k, f->n,
                        f->capacity, (const V4<T>*)f->pos, (const T*)f->mass,
