@@ -51,6 +51,11 @@ enum { TSPH_DFSPH = 0, TSPH_VFSPH = 1 };
  * (sph_config.py:83-232): the host layer fills it from config.json and the
  * scenario's sim_env with the same f64->f32 rounding points.
  */
+/* TsphParams.flags */
+#define TSPH_FLAG_FAST_QUOTIENT 1 /* FP32 fluid pass A may use the reduced sqrt and r/h
+                                     sequences: set only after tsph_verify_fast_quotient
+                                     found no difference for these parameters */
+
 typedef struct TsphParams {
   int32_t dim, phase_num, solver_type, precision;
   int32_t neighb_range, ntemp, node_num, list_cap_f;
@@ -111,6 +116,18 @@ typedef struct TsphControl {
 const char* tsph_last_error(void);
 int tsph_abi_version(void);
 size_t tsph_control_bytes(void);
+
+/*
+ * Sweeps every FP32 r^2 in [0, r2_max] and writes into mismatches[3] (device
+ * int32, overwritten): [0] the inputs for which pass A's reduced sqrt and
+ * r / h sequences give a different W(q) or r > 0 than the IEEE operations,
+ * for these parameters' h; [1] those with a different W'(q); [2] the largest
+ * r^2 bit pattern counted in [0].  No counterpart in the reference: it is the
+ * check behind TSPH_FLAG_FAST_QUOTIENT (set it only when [0] is 0), which
+ * keeps pass A's strict sums bit-identical.  FP64 parameters write 1, 1, 0
+ * (the flag is FP32 only).
+ */
+int tsph_verify_fast_quotient(const TsphParams* p, int32_t* mismatches, void* stream);
 
 /* Workspace needed by the calls below for an object of `capacity` slots. */
 size_t tsph_workspace_bytes(const TsphParams* p, int32_t capacity);

@@ -214,6 +214,32 @@ def test_fp64_fused_step_matches_fp64_oracle():
     compare_fields(sim, o, ["pos", "vel", "sph_density", "alpha", "psi_adv"], tol=1e-10, label="fp64")
 
 
+@pytest.mark.parametrize("name", ["dambreak_2d_8k", "dambreak_3d_8k", "two_phase_3d_4k"])
+def test_fast_quotient_is_verified_and_bit_identical(name):
+    """Pass A's reduced sqrt and r / h sequences are enabled only after the
+    device sweep over every FP32 r^2 in [0, r2_max] found no difference in
+    W, W' or r > 0; with them on or off, two fused steps give bit-identical
+    states."""
+    import ctypes
+    from tichi_sph_amd import _lib
+    a, _ = make(name)
+    b, _ = make(name, fast_quotient=False)
+    out = torch.zeros(3, dtype=torch.int32, device="cuda")
+    _lib.check(a.lib.tsph_verify_fast_quotient(ctypes.byref(b.params), ctypes.c_void_p(out.data_ptr()),
+                                               b.stream_ptr()), "verify")
+    w_or_r, wg, last = out.tolist()
+    print(f"  {name}: W/r>0 differ {w_or_r}, W' differ {wg}, largest W/r>0 r^2 bits {last:#x}", flush=True)
+    assert w_or_r == 0
+    assert a.params.flags & _lib.TSPH_FLAG_FAST_QUOTIENT
+    assert not b.params.flags & _lib.TSPH_FLAG_FAST_QUOTIENT
+    for _ in range(2):
+        assert a.sph_step() == b.sph_step()
+    for f in ("pos", "vel", "W", "sph_density", "sph_compression", "alpha", "alpha_1", "alpha_2",
+              "psi_adv"):
+        for oa, ob in ((a.fluid, b.fluid), (a.bound, b.bound)):
+            assert np.array_equal(oa.to_numpy(f), ob.to_numpy(f)), f
+
+
 def test_two_phase_fbm_surface_tension_step():
     sim, o = make("two_phase_3d_4k", fbm=True)
     rng = np.random.default_rng(0)

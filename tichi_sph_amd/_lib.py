@@ -10,6 +10,7 @@ import os
 TSPH_MAX_PHASE = 8
 TSPH_MAX_ITER_LOG = 256
 ABI_VERSION = 2  # TSPH_ABI_VERSION of include/tichi_sph.h
+TSPH_FLAG_FAST_QUOTIENT = 1  # TsphParams.flags
 TSPH_DFSPH = 0
 TSPH_VFSPH = 1
 
@@ -100,6 +101,7 @@ EXPORTED = [
     "tsph_last_error", "tsph_abi_version", "tsph_control_bytes", "tsph_workspace_bytes",
     "tsph_sort_object", "tsph_ngrid_build", "tsph_neighbor_lists", "tsph_density_alpha",
     "tsph_solver_begin", "tsph_solver_begin_signal", "tsph_signal_alloc", "tsph_signal_free",
+    "tsph_verify_fast_quotient",
     "tsph_solver_iteration", "tsph_solver_psi", "tsph_solver_update", "tsph_advect",
     "tsph_fbm_round", "tsph_tail",
     "tsph_ref_kernel", "tsph_solver_psi_taped", "tsph_relist", "tsph_adj_begin",
@@ -151,6 +153,7 @@ def load(path=None):
         "tsph_solver_begin_signal": [P, C, _i32, _vp, _i32, _vp],
         "tsph_signal_alloc": [ctypes.c_size_t, ctypes.POINTER(_vp), ctypes.POINTER(_vp)],
         "tsph_signal_free": [_vp],
+        "tsph_verify_fast_quotient": [P, _vp, _vp],
         "tsph_solver_iteration": [P, O, O, C, _i32, _vp, sz, _vp],
         "tsph_solver_psi": [P, O, O, C, _i32, _vp, sz, _vp],
         "tsph_solver_update": [P, O, O, C, _i32, _vp, sz, _vp],

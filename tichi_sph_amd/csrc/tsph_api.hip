@@ -28,6 +28,7 @@ int density_alpha(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, voi
                   hipStream_t);
 template <class T>
 int solver_begin(const TsphParams&, TsphControl*, int, int*, int, hipStream_t);
+int verify_fast_quotient(const TsphParams&, int*, hipStream_t);
 template <class T>
 int solver_half(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, int, void*, void*,
                 void*, size_t, hipStream_t);
@@ -201,6 +202,12 @@ extern "C" {
 const char* tsph_last_error(void) { return g_err.c_str(); }
 int tsph_abi_version(void) { return TSPH_ABI_VERSION; }
 size_t tsph_control_bytes(void) { return sizeof(TsphControl); }
+
+int tsph_verify_fast_quotient(const TsphParams* p, int32_t* mismatches, void* stream) {
+  if (int rc = check_params(p)) return rc;
+  TSPH_REQUIRE(mismatches, TSPH_EINVAL, "tsph_verify_fast_quotient: null output");
+  return tsph::verify_fast_quotient(*p, (int*)mismatches, (hipStream_t)stream);
+}
 
 int tsph_signal_alloc(size_t bytes, void** host, void** device) {
   TSPH_REQUIRE(host && device && bytes > 0, TSPH_EINVAL, "tsph_signal_alloc: bad arguments");

@@ -34,6 +34,7 @@ struct KP {
   int gs[3], coder[3];
   int list_cap_f, list_cap_b;
   int iter_min, iter_max, cs, wc_gamma;
+  int fastq;     // FP32: q = r / h and sqrt by the verified reduced sequences (f_quot_h)
   T lb[3], g[3], rho[kMaxP];
   T h, h2, sig, sig_h, gps, c_coh, eps, r2max, inv_h;
   T mu, gamma, fbmD, fbmC, cfl;
@@ -90,11 +91,10 @@ __device__ __forceinline__ T s_pow3(T x) {
   return x * t;
 }
 
-// W, sph_util.py:16-25
+// W, sph_util.py:16-25, from q = r / h
 template <class T>
-__device__ __forceinline__ T s_W(const KP<T>& k, T r) {
+__device__ __forceinline__ T s_W_q(const KP<T>& k, T q) {
   TSPH_STRICT
-  T q = r / k.h;
   T tmp = T(0);
   if (q <= T(0.5)) {
     T q3 = s_pow3(q);
@@ -105,12 +105,16 @@ __device__ __forceinline__ T s_W(const KP<T>& k, T r) {
   }
   return tmp * k.sig;
 }
-
-// W_grad, sph_util.py:28-37
 template <class T>
-__device__ __forceinline__ T s_Wg(const KP<T>& k, T r) {
+__device__ __forceinline__ T s_W(const KP<T>& k, T r) {
   TSPH_STRICT
-  T q = r / k.h;
+  return s_W_q(k, r / k.h);
+}
+
+// W_grad, sph_util.py:28-37, from q = r / h
+template <class T>
+__device__ __forceinline__ T s_Wg_q(const KP<T>& k, T q) {
+  TSPH_STRICT
   T tmp = T(0);
   if (q <= T(0.5)) {
     T q2 = q * q;
@@ -122,6 +126,62 @@ __device__ __forceinline__ T s_Wg(const KP<T>& k, T r) {
     tmp = T(-6) * (u * u);
   }
   return tmp * k.sig_h;
+}
+template <class T>
+__device__ __forceinline__ T s_Wg(const KP<T>& k, T r) {
+  TSPH_STRICT
+  return s_Wg_q(k, r / k.h);
+}
+
+// ---------------------------------------------------------------------------
+// Reduced sequences for pass A's r = sqrt(r^2) and q = r / h.
+//   quotient: q0 = r * RN(1/h), then one residual correction
+//             q = q0 + (r - q0 h) RN(1/h), the residual exact by FMA
+//             (Markstein's correction step);
+//   sqrt:     the hardware square root (within 1 ulp) followed by the
+//             residual test of its two neighbours, without the IEEE
+//             sequence's rescaling of tiny inputs.
+// The fluid half of pass A reads r only through W(q) and the test r > 0.  A
+// sweep over every FP32 r^2 in [0, r2_max] (k_verify_fast_quotient) compares
+// those results of the reduced and the IEEE sequences bit for bit for this
+// h; the host sets TSPH_FLAG_FAST_QUOTIENT only when none differs, so the
+// strict sums stay bit-identical.  (The boundary half also reads W'(q),
+// which differs for tiny r, and keeps the IEEE sequences.)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float f_quot_h(float r, float h, float inv_h) {
+  const float q0 = r * inv_h;
+  const float e = __builtin_fmaf(-q0, h, r);
+  return __builtin_fmaf(e, inv_h, q0);
+}
+__device__ __forceinline__ float f_sqrt_cr(float x) {
+  float s = __builtin_amdgcn_sqrtf(x);
+  const float sm = __int_as_float(__float_as_int(s) - 1);
+  const float sp = __int_as_float(__float_as_int(s) + 1);
+  const float rm = __builtin_fmaf(-sm, s, x);
+  const float rp = __builtin_fmaf(-sp, s, x);
+  s = rm <= 0.0f ? sm : s;
+  s = rp > 0.0f ? sp : s;
+  return s;
+}
+
+// Pass A's sqrt and q: the IEEE operations, or (FQ, FP32 only) the reduced
+// sequences above.
+template <bool FQ, class T>
+__device__ __forceinline__ T pa_sqrt(T x) {
+  return s_sqrt(x);
+}
+template <>
+__device__ __forceinline__ float pa_sqrt<true, float>(float x) {
+  return f_sqrt_cr(x);
+}
+template <bool FQ, class T>
+__device__ __forceinline__ T pa_quot(const KP<T>& k, T r) {
+  TSPH_STRICT
+  return r / k.h;
+}
+template <>
+__device__ __forceinline__ float pa_quot<true, float>(const KP<float>& k, float r) {
+  return f_quot_h(r, k.h, k.inv_h);
 }
 
 // C, sph_util.py:4-13

@@ -9,6 +9,9 @@
 // (0 < r < h, exact f32 test) of each fluid particle in reference order;
 // every later sweep iterates only those lists.  Terms beyond h are exact
 // zeros in the reference, so skipping them changes no value.
+#include <cstring>
+#include <type_traits>
+
 #include "tsph_sweep.h"
 
 namespace tsph {
@@ -18,6 +21,7 @@ namespace tsph {
 template <class T>
 struct PassAAcc {
   T w = 0, cp = 0, dn = 0, ax = 0, ay = 0, az = 0, a2 = 0;
+  template <bool FQ>
   __device__ __forceinline__ bool add(const KP<T>& k, const V4<T>& p, const V4<T>& q, T mj, T vj,
                                       bool fluid) {
     T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
@@ -25,8 +29,8 @@ struct PassAAcc {
     {
       TSPH_STRICT
       r2 = s_r2(dx, dy, dz);
-      T r = s_sqrt(r2);
-      T Wr = s_W(k, r);
+      T r = pa_sqrt<FQ>(r2);
+      T Wr = s_W_q(k, pa_quot<FQ>(k, r));
       w = w + Wr;
       cp = cp + Wr * vj;
       dn = dn + Wr * mj;
@@ -86,7 +90,7 @@ __device__ __forceinline__ void pass_a_epilogue(const KP<T>& k, int i, const V4<
   nbr_nb[i] = nbb;
 }
 
-template <class T>
+template <class T, bool FQ>
 __device__ __forceinline__ void k_pass_a_fluid_body(Blk blk, KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
                    const T* __restrict__ fmass, const T* __restrict__ frvol,
                    const int* __restrict__ fstart, int nb, const V4<T>* __restrict__ bpos,
@@ -109,14 +113,14 @@ __device__ __forceinline__ void k_pass_a_fluid_body(Blk blk, KP<T> k, int n, int
   int nf = 0, nbb = 0;
   walk_cells_batched<kPassAU>(k, fstart, c, fpos, [&](int j, const V4<T>& q) {
     if (s_r2(p.x - q.x, p.y - q.y, p.z - q.z) <= k.r2max &&
-        acc.add(k, p, q, fmass[j], frvol[j], true)) {
+        acc.template add<FQ>(k, p, q, fmass[j], frvol[j], true)) {
       if (nf < capf) nbr_f[(long)nf * cap + i] = j;
       ++nf;
     }
   });
   walk_cells_batched<kPassAU>(k, bstart, c, bpos, [&](int j, const V4<T>& q) {
     if (s_r2(p.x - q.x, p.y - q.y, p.z - q.z) <= k.r2max &&
-        acc.add(k, p, q, bmass[j], brvol[j], false)) {
+        acc.template add<FQ>(k, p, q, bmass[j], brvol[j], false)) {
       if (nbb < capb) nbr_b[(long)nbb * cap + i] = j;
       ++nbb;
     }
@@ -137,12 +141,16 @@ __global__ void __launch_bounds__(kBlock)
                    V4<T>* __restrict__ vel_adv, int* __restrict__ nbr_f, int* __restrict__ nbr_b,
                    int* __restrict__ nbr_nf, int* __restrict__ nbr_nb, int* __restrict__ flag,
                    TsphControl* ctrl) {
-  k_pass_a_fluid_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, n, cap, fpos, fmass, frvol, fstart, nb, bpos, bmass, brvol, bstart, W, comp, dens, alpha_1, alpha_2, alpha, vol, vel, vel_adv, nbr_f, nbr_b, nbr_nf, nbr_nb, flag, ctrl);
+  const Blk blk{(int)blockIdx.x, (int)gridDim.x};
+  if (k.fastq)
+    k_pass_a_fluid_body<T, true>(blk, k, n, cap, fpos, fmass, frvol, fstart, nb, bpos, bmass, brvol, bstart, W, comp, dens, alpha_1, alpha_2, alpha, vol, vel, vel_adv, nbr_f, nbr_b, nbr_nf, nbr_nb, flag, ctrl);
+  else
+    k_pass_a_fluid_body<T, false>(blk, k, n, cap, fpos, fmass, frvol, fstart, nb, bpos, bmass, brvol, bstart, W, comp, dens, alpha_1, alpha_2, alpha, vol, vel, vel_adv, nbr_f, nbr_b, nbr_nf, nbr_nb, flag, ctrl);
 }
 
 // Pass A, boundary particles: SPH_prepare_attr (b,b)(b,f), SPH_prepare_alpha_2
 // (b,f), SPH_prepare_alpha (alpha_1 of the boundary is never accumulated).
-template <class T>
+template <class T, bool FQ>
 __device__ __forceinline__ void k_pass_a_bound_body(Blk blk, KP<T> k, int nb, const V4<T>* __restrict__ bpos, const T* __restrict__ bmass,
                    const T* __restrict__ brvol, const int* __restrict__ bstart,
                    const V4<T>* __restrict__ fpos, const T* __restrict__ fmass,
@@ -160,7 +168,7 @@ __device__ __forceinline__ void k_pass_a_bound_body(Blk blk, KP<T> k, int nb, co
   walk_cells_batched<kPassAU>(k, bstart, c, bpos, [&](int j, const V4<T>& q) {
     T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
     if (r2 <= k.r2max) {
-      T Wr = s_W(k, s_sqrt(r2));
+      T Wr = s_W_q(k, pa_quot<FQ>(k, pa_sqrt<FQ>(r2)));
       w = w + Wr;
       cp = cp + Wr * brvol[j];
       dn = dn + Wr * bmass[j];
@@ -169,13 +177,14 @@ __device__ __forceinline__ void k_pass_a_bound_body(Blk blk, KP<T> k, int nb, co
   walk_cells_batched<kPassAU>(k, fstart, c, fpos, [&](int j, const V4<T>& q) {
     T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
     if (r2 <= k.r2max) {
-      T r = s_sqrt(r2);
-      T Wr = s_W(k, r);
+      T r = pa_sqrt<FQ>(r2);
+      T qh = pa_quot<FQ>(k, r);
+      T Wr = s_W_q(k, qh);
       w = w + Wr;
       cp = cp + Wr * frvol[j];
       dn = dn + Wr * fmass[j];
       if (r > T(0)) {
-        T wg = s_Wg(k, r);
+        T wg = s_Wg_q(k, qh);
         T Xj = q.w;
         T wg2 = wg * wg;
         a2 = a2 + wg2 * (Xj * Xj) / fmass[j];
@@ -203,7 +212,7 @@ __global__ void __launch_bounds__(kBlock)
                    T* __restrict__ W, T* __restrict__ comp, T* __restrict__ dens,
                    V4<T>* __restrict__ alpha_1, T* __restrict__ alpha_2, T* __restrict__ alpha,
                    T* __restrict__ vol) {
-  k_pass_a_bound_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, nb, bpos, bmass, brvol, bstart, fpos, fmass, frvol, fstart, W, comp, dens, alpha_1, alpha_2, alpha, vol);
+  k_pass_a_bound_body<T, false>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, nb, bpos, bmass, brvol, bstart, fpos, fmass, frvol, fstart, W, comp, dens, alpha_1, alpha_2, alpha, vol);
 }
 
 // Per-step reset of the control block's step-scoped state: general_flag = 1
@@ -222,25 +231,95 @@ __global__ void k_step_reset(TsphControl* ctrl) { step_reset_body(ctrl); }
 // the rest the boundary body.  The two halves are independent (each writes
 // only its own object's fields), so the boundary blocks fill the tail of the
 // fluid's grid instead of waiting for it.
+template <class T, bool FQ>
+__device__ __forceinline__ void pass_a_both_body(const KP<T>& k, const TsphObject& f,
+                                                 const TsphObject& b, int nbf, TsphControl* ctrl) {
+  if ((int)blockIdx.x < nbf) {
+    k_pass_a_fluid_body<T, FQ>(Blk{(int)blockIdx.x, nbf}, k, f.n, f.capacity, (const V4<T>*)f.pos,
+                               (const T*)f.mass, (const T*)f.rest_volume, f.cell_start, b.n,
+                               (const V4<T>*)b.pos, (const T*)b.mass, (const T*)b.rest_volume,
+                               b.cell_start, (T*)f.W, (T*)f.sph_compression, (T*)f.sph_density,
+                               (V4<T>*)f.alpha_1, (T*)f.alpha_2, (T*)f.alpha, (T*)f.vol,
+                               (const V4<T>*)f.vel, (V4<T>*)f.vel_adv, f.nbr_f, f.nbr_b, f.nbr_nf,
+                               f.nbr_nb, f.flag, ctrl);
+  } else {
+    k_pass_a_bound_body<T, false>(Blk{(int)blockIdx.x - nbf, (int)gridDim.x - nbf}, k, b.n,
+                               (const V4<T>*)b.pos, (const T*)b.mass, (const T*)b.rest_volume,
+                               b.cell_start, (const V4<T>*)f.pos, (const T*)f.mass,
+                               (const T*)f.rest_volume, f.cell_start, (T*)b.W,
+                               (T*)b.sph_compression, (T*)b.sph_density, (V4<T>*)b.alpha_1,
+                               (T*)b.alpha_2, (T*)b.alpha, (T*)b.vol);
+  }
+}
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_pass_a_both(KP<T> k, TsphObject f, TsphObject b, int nbf, TsphControl* ctrl) {
-  if ((int)blockIdx.x < nbf) {
-    k_pass_a_fluid_body<T>(Blk{(int)blockIdx.x, nbf}, k, f.n, f.capacity, (const V4<T>*)f.pos,
-                           (const T*)f.mass, (const T*)f.rest_volume, f.cell_start, b.n,
-                           (const V4<T>*)b.pos, (const T*)b.mass, (const T*)b.rest_volume,
-                           b.cell_start, (T*)f.W, (T*)f.sph_compression, (T*)f.sph_density,
-                           (V4<T>*)f.alpha_1, (T*)f.alpha_2, (T*)f.alpha, (T*)f.vol,
-                           (const V4<T>*)f.vel, (V4<T>*)f.vel_adv, f.nbr_f, f.nbr_b, f.nbr_nf,
-                           f.nbr_nb, f.flag, ctrl);
-  } else {
-    k_pass_a_bound_body<T>(Blk{(int)blockIdx.x - nbf, (int)gridDim.x - nbf}, k, b.n,
-                           (const V4<T>*)b.pos, (const T*)b.mass, (const T*)b.rest_volume,
-                           b.cell_start, (const V4<T>*)f.pos, (const T*)f.mass,
-                           (const T*)f.rest_volume, f.cell_start, (T*)b.W,
-                           (T*)b.sph_compression, (T*)b.sph_density, (V4<T>*)b.alpha_1,
-                           (T*)b.alpha_2, (T*)b.alpha, (T*)b.vol);
+  if (k.fastq)
+    pass_a_both_body<T, true>(k, f, b, nbf, ctrl);
+  else
+    pass_a_both_body<T, false>(k, f, b, nbf, ctrl);
+}
+
+// The check behind TSPH_FLAG_FAST_QUOTIENT: for every FP32 r^2 in
+// [0, r2_max] (bit patterns 0 .. umax), the reduced and the IEEE sequences
+// are compared on W(q) and r > 0 (out[0]: inputs that differ, out[2]: the
+// largest such bit pattern) and on W'(q) (out[1]).  Each wave adds its
+// counts once.
+__global__ void __launch_bounds__(kBlock) k_verify_fast_quotient(KP<float> k, uint32_t umax, int* out) {
+  int bad = 0, badg = 0;
+  uint32_t last = 0;
+  const uint32_t stride = gridDim.x * kBlock;
+  for (uint32_t u = blockIdx.x * kBlock + threadIdx.x; u <= umax; u += stride) {
+    const float x = __uint_as_float(u);
+    float rs, ws, gs;
+    {
+      TSPH_STRICT
+      rs = s_sqrt(x);
+      const float qs = rs / k.h;
+      ws = s_W_q(k, qs);
+      gs = s_Wg_q(k, qs);
+    }
+    const float rf = f_sqrt_cr(x);
+    const float qf = f_quot_h(rf, k.h, k.inv_h);
+    const float wf = s_W_q(k, qf), gf = s_Wg_q(k, qf);
+    const bool dw = (__float_as_uint(ws) != __float_as_uint(wf)) || ((rs > 0.0f) != (rf > 0.0f));
+    bad += dw;
+    badg += __float_as_uint(gs) != __float_as_uint(gf);
+    if (dw) last = u;
   }
+  for (int o = 32; o > 0; o >>= 1) {
+    bad += __shfl_xor(bad, o, 64);
+    badg += __shfl_xor(badg, o, 64);
+    last = max(last, (uint32_t)__shfl_xor((int)last, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (bad) atomicAdd(out, bad);
+    if (badg) atomicAdd(out + 1, badg);
+    if (last) atomicMax((unsigned int*)(out + 2), last);
+  }
+}
+
+__global__ void k_fill3_i32(int* p, int v0, int v1, int v2) {
+  p[0] = v0;
+  p[1] = v1;
+  p[2] = v2;
+}
+
+int verify_fast_quotient(const TsphParams& prm, int* out, hipStream_t st) {
+  if (prm.precision != 4) {
+    hipLaunchKernelGGL(k_fill3_i32, dim3(1), dim3(1), 0, st, out, 1, 1, 0);
+    TSPH_LAUNCH_CHECK("verify_fast_quotient");
+    return 0;
+  }
+  KP<float> k = make_kp<float>(prm);
+  uint32_t umax;
+  std::memcpy(&umax, &k.r2max, sizeof(umax));
+  TSPH_REQUIRE(k.r2max >= 0.0f && umax < 0x7f800000u, TSPH_EINVAL,
+               "tsph_verify_fast_quotient: r2_max is not a finite non-negative value");
+  hipLaunchKernelGGL(k_fill3_i32, dim3(1), dim3(1), 0, st, out, 0, 0, 0);
+  hipLaunchKernelGGL(k_verify_fast_quotient, dim3(8192), dim3(kBlock), 0, st, k, umax, out);
+  TSPH_LAUNCH_CHECK("verify_fast_quotient");
+  return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1080,13 +1159,19 @@ __global__ void __launch_bounds__(kBlock) kb_pass_a_fluid(const char* __restrict
   for_tiles<T, kTileFluid>(tab, ns, [&](const SceneEntry<T>& e, int, Blk blk) {
     const TsphObject& f = e.f;
     const TsphObject& b = e.b;
-    k_pass_a_fluid_body<T>(blk, e.k, f.n, f.capacity, (const V4<T>*)f.pos, (const T*)f.mass,
-                           (const T*)f.rest_volume, f.cell_start, b.n, (const V4<T>*)b.pos,
-                           (const T*)b.mass, (const T*)b.rest_volume, b.cell_start, (T*)f.W,
-                           (T*)f.sph_compression, (T*)f.sph_density, (V4<T>*)f.alpha_1,
-                           (T*)f.alpha_2, (T*)f.alpha, (T*)f.vol, (const V4<T>*)f.vel,
-                           (V4<T>*)f.vel_adv, f.nbr_f, f.nbr_b, f.nbr_nf, f.nbr_nb, f.flag,
-                           e.ctrl);
+    auto run = [&](auto fq) {
+      k_pass_a_fluid_body<T, decltype(fq)::value>(
+          blk, e.k, f.n, f.capacity, (const V4<T>*)f.pos, (const T*)f.mass,
+          (const T*)f.rest_volume, f.cell_start, b.n, (const V4<T>*)b.pos, (const T*)b.mass,
+          (const T*)b.rest_volume, b.cell_start, (T*)f.W, (T*)f.sph_compression,
+          (T*)f.sph_density, (V4<T>*)f.alpha_1, (T*)f.alpha_2, (T*)f.alpha, (T*)f.vol,
+          (const V4<T>*)f.vel, (V4<T>*)f.vel_adv, f.nbr_f, f.nbr_b, f.nbr_nf, f.nbr_nb, f.flag,
+          e.ctrl);
+    };
+    if (e.k.fastq)
+      run(std::true_type{});
+    else
+      run(std::false_type{});
   });
 }
 
@@ -1095,11 +1180,11 @@ __global__ void __launch_bounds__(kBlock) kb_pass_a_bound(const char* __restrict
   for_tiles<T, kTileBound>(tab, ns, [&](const SceneEntry<T>& e, int, Blk blk) {
     const TsphObject& f = e.f;
     const TsphObject& b = e.b;
-    k_pass_a_bound_body<T>(blk, e.k, b.n, (const V4<T>*)b.pos, (const T*)b.mass,
-                           (const T*)b.rest_volume, b.cell_start, (const V4<T>*)f.pos,
-                           (const T*)f.mass, (const T*)f.rest_volume, f.cell_start, (T*)b.W,
-                           (T*)b.sph_compression, (T*)b.sph_density, (V4<T>*)b.alpha_1,
-                           (T*)b.alpha_2, (T*)b.alpha, (T*)b.vol);
+    k_pass_a_bound_body<T, false>(
+        blk, e.k, b.n, (const V4<T>*)b.pos, (const T*)b.mass, (const T*)b.rest_volume,
+        b.cell_start, (const V4<T>*)f.pos, (const T*)f.mass, (const T*)f.rest_volume,
+        f.cell_start, (T*)b.W, (T*)b.sph_compression, (T*)b.sph_density, (V4<T>*)b.alpha_1,
+        (T*)b.alpha_2, (T*)b.alpha, (T*)b.vol);
   });
 }
 

@@ -82,6 +82,24 @@ class EventTimer:
         return out
 
 
+_FASTQ_CACHE = {}
+
+
+def _fast_quotient_ok(lib, params, device):
+    """Whether pass A's reduced sqrt and r / h sequences give bit-identical
+    W and r > 0 for these parameters (tsph_verify_fast_quotient sweeps every
+    FP32 r^2 in [0, r2_max] on the device, about 2 ms); cached per kernel
+    constants."""
+    key = (params.dim, tuple(params.kernel_h), tuple(params.kernel_sig), params.r2_max)
+    if key not in _FASTQ_CACHE:
+        out = torch.zeros(3, dtype=torch.int32, device=device)
+        st = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+        _lib.check(lib.tsph_verify_fast_quotient(ctypes.byref(params), ctypes.c_void_p(out.data_ptr()), st),
+                   "tsph_verify_fast_quotient")
+        _FASTQ_CACHE[key] = int(out[0].item()) == 0
+    return _FASTQ_CACHE[key]
+
+
 class GlobalVariable:
     """The run-time counters of sph_obj.py:347-369 that the step loop and the
     writers use.  step_counter counts sph_steps (the reference declares it
@@ -124,7 +142,7 @@ class Simulation:
     GPU only)."""
 
     def __init__(self, config, device="cuda", precision=4, fbm=False, wcsph=False, cfl=True,
-                 list_cap_f=64, list_cap_b=48, fluid_max=None, bound_max=None):
+                 list_cap_f=64, list_cap_b=48, fluid_max=None, bound_max=None, fast_quotient=True):
         self.config = config
         self.device = torch.device(device)
         self.precision = precision
@@ -133,6 +151,9 @@ class Simulation:
         self.wcsph = wcsph
         self.cfl = cfl
         self.params = config.params(precision, list_cap_f, list_cap_b)
+        if fast_quotient and precision == 4 and self.device.type == "cuda":
+            if _fast_quotient_ok(self.lib, self.params, self.device):
+                self.params.flags |= _lib.TSPH_FLAG_FAST_QUOTIENT
         fm = config.fluid_max_part_num if fluid_max is None else fluid_max
         bm = config.bound_max_part_num if bound_max is None else bound_max
         self.fluid = Fluid(config, fm, 0, device, precision, list_cap_f, list_cap_b)
