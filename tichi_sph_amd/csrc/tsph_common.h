@@ -110,6 +110,21 @@ __device__ __forceinline__ T s_W(const KP<T>& k, T r) {
   TSPH_STRICT
   return s_W_q(k, r / k.h);
 }
+// s_W_q with both pieces evaluated and one selected: the same operations,
+// so the same value, without a divergent branch per pair.  The empty asm
+// statement only pins both pieces as computed (it emits no instruction), so
+// that the compiler does not turn the select back into branches.
+template <class T>
+__device__ __forceinline__ T s_W_q_sel(const KP<T>& k, T q) {
+  TSPH_STRICT
+  T q3 = s_pow3(q);
+  T q2 = q * q;
+  T a = T(6) * (q3 - q2) + T(1);
+  T b = T(2) * s_pow3(T(1) - q);
+  __asm__("" : "+v"(a), "+v"(b));
+  T tmp = q <= T(0.5) ? a : (q < T(1) ? b : T(0));
+  return tmp * k.sig;
+}
 
 // W_grad, sph_util.py:28-37, from q = r / h
 template <class T>

@@ -30,7 +30,7 @@ struct PassAAcc {
       TSPH_STRICT
       r2 = s_r2(dx, dy, dz);
       T r = pa_sqrt<FQ>(r2);
-      T Wr = s_W_q(k, pa_quot<FQ>(k, r));
+      T Wr = s_W_q_sel(k, pa_quot<FQ>(k, r));
       w = w + Wr;
       cp = cp + Wr * vj;
       dn = dn + Wr * mj;
