@@ -128,6 +128,7 @@ struct WsLayout {
   int* cu_start;       // [2*node_num + 3]
   void* krec;          // [cap] V4 of the step's precision: position and kappa of
                        // each fluid particle in the current Jacobi iteration
+  void* a1b;           // [cap] V4: boundary part of alpha_1 (sum_b X_j grad W_ij), pass A
 };
 
 inline size_t ws_bytes_needed(long cap, long node_num) {
@@ -135,7 +136,7 @@ inline size_t ws_bytes_needed(long cap, long node_num) {
   long lenref = 2 * node_num + 2;
   return seg(sizeof(double) * 2 * maxblk) + seg(sizeof(int) * (scan_blocks(lenref + 1) + 8)) +
          5 * seg(sizeof(int) * cap) + seg(sizeof(int) * (lenref + 1)) +
-         seg(sizeof(int) * (lenref + 2)) + seg(4 * sizeof(double) * cap) + 4096;
+         seg(sizeof(int) * (lenref + 2)) + 2 * seg(4 * sizeof(double) * cap) + 4096;
 }
 
 inline bool carve_ws(void* ws, size_t bytes, long cap, long node_num, WsLayout* L) {
@@ -152,6 +153,7 @@ inline bool carve_ws(void* ws, size_t bytes, long cap, long node_num, WsLayout* 
   L->cu_count = c.take<int>(lenref + 1);
   L->cu_start = c.take<int>(lenref + 2);
   L->krec = c.take<double>(4 * cap);
+  L->a1b = c.take<double>(4 * cap);
   return c.ok;
 }
 

@@ -21,6 +21,7 @@ namespace tsph {
 template <class T>
 struct PassAAcc {
   T w = 0, cp = 0, dn = 0, ax = 0, ay = 0, az = 0, a2 = 0;
+  T bx = 0, by = 0, bz = 0;  // the boundary neighbours' share of alpha_1
   template <bool FQ>
   __device__ __forceinline__ bool add(const KP<T>& k, const V4<T>& p, const V4<T>& q, T mj, T vj,
                                       bool fluid) {
@@ -43,7 +44,13 @@ struct PassAAcc {
     ax += xg * dx;
     ay += xg * dy;
     az += xg * dz;
-    if (fluid) a2 += (g * g) * r2 * (Xj * Xj) * f_rcp(mj);
+    if (fluid) {
+      a2 += (g * g) * r2 * (Xj * Xj) * f_rcp(mj);
+    } else {
+      bx += xg * dx;
+      by += xg * dy;
+      bz += xg * dz;
+    }
     return true;
   }
 };
@@ -100,7 +107,7 @@ __device__ __forceinline__ void k_pass_a_fluid_body(Blk blk, KP<T> k, int n, int
                    T* __restrict__ alpha, T* __restrict__ vol, const V4<T>* __restrict__ vel,
                    V4<T>* __restrict__ vel_adv, int* __restrict__ nbr_f, int* __restrict__ nbr_b,
                    int* __restrict__ nbr_nf, int* __restrict__ nbr_nb, int* __restrict__ flag,
-                   TsphControl* ctrl) {
+                   V4<T>* __restrict__ a1b, TsphControl* ctrl) {
   const int i = xcd_block(blk.b, blk.nb) * kBlock + threadIdx.x;
   if (i >= n) return;
   const int capf = k.list_cap_f, capb = k.list_cap_b;
@@ -125,6 +132,7 @@ __device__ __forceinline__ void k_pass_a_fluid_body(Blk blk, KP<T> k, int n, int
       ++nbb;
     }
   });
+  a1b[i] = V4<T>{acc.bx, acc.by, acc.bz, T(0)};
   pass_a_epilogue(k, i, p, mi, acc, nf, nbb, W, comp, dens, alpha_1, alpha_2, alpha, vol, vel,
                   vel_adv, nbr_nf, nbr_nb, ctrl);
 }
@@ -140,12 +148,12 @@ __global__ void __launch_bounds__(kBlock)
                    T* __restrict__ alpha, T* __restrict__ vol, const V4<T>* __restrict__ vel,
                    V4<T>* __restrict__ vel_adv, int* __restrict__ nbr_f, int* __restrict__ nbr_b,
                    int* __restrict__ nbr_nf, int* __restrict__ nbr_nb, int* __restrict__ flag,
-                   TsphControl* ctrl) {
+                   V4<T>* __restrict__ a1b, TsphControl* ctrl) {
   const Blk blk{(int)blockIdx.x, (int)gridDim.x};
   if (k.fastq)
-    k_pass_a_fluid_body<T, true>(blk, k, n, cap, fpos, fmass, frvol, fstart, nb, bpos, bmass, brvol, bstart, W, comp, dens, alpha_1, alpha_2, alpha, vol, vel, vel_adv, nbr_f, nbr_b, nbr_nf, nbr_nb, flag, ctrl);
+    k_pass_a_fluid_body<T, true>(blk, k, n, cap, fpos, fmass, frvol, fstart, nb, bpos, bmass, brvol, bstart, W, comp, dens, alpha_1, alpha_2, alpha, vol, vel, vel_adv, nbr_f, nbr_b, nbr_nf, nbr_nb, flag, a1b, ctrl);
   else
-    k_pass_a_fluid_body<T, false>(blk, k, n, cap, fpos, fmass, frvol, fstart, nb, bpos, bmass, brvol, bstart, W, comp, dens, alpha_1, alpha_2, alpha, vol, vel, vel_adv, nbr_f, nbr_b, nbr_nf, nbr_nb, flag, ctrl);
+    k_pass_a_fluid_body<T, false>(blk, k, n, cap, fpos, fmass, frvol, fstart, nb, bpos, bmass, brvol, bstart, W, comp, dens, alpha_1, alpha_2, alpha, vol, vel, vel_adv, nbr_f, nbr_b, nbr_nf, nbr_nb, flag, a1b, ctrl);
 }
 
 // Pass A, boundary particles: SPH_prepare_attr (b,b)(b,f), SPH_prepare_alpha_2
@@ -233,7 +241,8 @@ __global__ void k_step_reset(TsphControl* ctrl) { step_reset_body(ctrl); }
 // fluid's grid instead of waiting for it.
 template <class T, bool FQ>
 __device__ __forceinline__ void pass_a_both_body(const KP<T>& k, const TsphObject& f,
-                                                 const TsphObject& b, int nbf, TsphControl* ctrl) {
+                                                 const TsphObject& b, int nbf, V4<T>* a1b,
+                                                 TsphControl* ctrl) {
   if ((int)blockIdx.x < nbf) {
     k_pass_a_fluid_body<T, FQ>(Blk{(int)blockIdx.x, nbf}, k, f.n, f.capacity, (const V4<T>*)f.pos,
                                (const T*)f.mass, (const T*)f.rest_volume, f.cell_start, b.n,
@@ -241,7 +250,7 @@ __device__ __forceinline__ void pass_a_both_body(const KP<T>& k, const TsphObjec
                                b.cell_start, (T*)f.W, (T*)f.sph_compression, (T*)f.sph_density,
                                (V4<T>*)f.alpha_1, (T*)f.alpha_2, (T*)f.alpha, (T*)f.vol,
                                (const V4<T>*)f.vel, (V4<T>*)f.vel_adv, f.nbr_f, f.nbr_b, f.nbr_nf,
-                               f.nbr_nb, f.flag, ctrl);
+                               f.nbr_nb, f.flag, a1b, ctrl);
   } else {
     k_pass_a_bound_body<T, false>(Blk{(int)blockIdx.x - nbf, (int)gridDim.x - nbf}, k, b.n,
                                (const V4<T>*)b.pos, (const T*)b.mass, (const T*)b.rest_volume,
@@ -253,11 +262,11 @@ __device__ __forceinline__ void pass_a_both_body(const KP<T>& k, const TsphObjec
 }
 template <class T>
 __global__ void __launch_bounds__(kBlock)
-    k_pass_a_both(KP<T> k, TsphObject f, TsphObject b, int nbf, TsphControl* ctrl) {
+    k_pass_a_both(KP<T> k, TsphObject f, TsphObject b, int nbf, V4<T>* a1b, TsphControl* ctrl) {
   if (k.fastq)
-    pass_a_both_body<T, true>(k, f, b, nbf, ctrl);
+    pass_a_both_body<T, true>(k, f, b, nbf, a1b, ctrl);
   else
-    pass_a_both_body<T, false>(k, f, b, nbf, ctrl);
+    pass_a_both_body<T, false>(k, f, b, nbf, a1b, ctrl);
 }
 
 // The check behind TSPH_FLAG_FAST_QUOTIENT: for every FP32 r^2 in
@@ -359,7 +368,8 @@ __device__ __forceinline__ void k_solver_psi_body(Blk blk, KP<T> k, int n, int c
                  const int* __restrict__ bstart, const int* __restrict__ nbr_f,
                  const int* __restrict__ nbr_b, const int* __restrict__ nbr_nf,
                  const int* __restrict__ nbr_nb, T* __restrict__ psi_adv, T* __restrict__ kappa,
-                 V4<T>* __restrict__ krec, V4<T>* __restrict__ tape_v, T* __restrict__ tape_k,
+                 V4<T>* __restrict__ krec, const V4<T>* __restrict__ a1b,
+                 V4<T>* __restrict__ tape_v, T* __restrict__ tape_k,
                  double* __restrict__ partial, TsphControl* ctrl) {
   __shared__ double lds[kWaves];
   if (!ctrl->run_u[kk]) return;
@@ -383,13 +393,17 @@ __device__ __forceinline__ void k_solver_psi_body(Blk blk, KP<T> k, int n, int c
       T dot = (dx * (va.x - vj.x) + dy * (va.y - vj.y) + dz * (va.z - vj.z)) * g;
       sum += dot * q.w;
     };
+    // (f,f) pairs from the list; the (f,b) pairs have v_j = 0 (boundary
+    // vel_adv is never written), so their sum is v_i . sum_b X_j grad W_ij,
+    // the boundary share of alpha_1 that pass A recorded
     sweep_lists_coop<kCoop, T, Rec2<T>>(
-        k, cap, i, c, p, fstart, bstart, fpos, bpos, nbr_f, nbr_b, nbr_nf[i], nbr_nb[i],
+        k, cap, i, c, p, fstart, bstart, fpos, bpos, nbr_f, nullptr, nbr_nf[i], 0,
         [&](int j) { return Rec2<T>{fpos[j], vadv[j]}; },
-        // boundary vel_adv is never written: 0
-        [&](int j) { return Rec2<T>{bpos[j], V4<T>{T(0), T(0), T(0), T(0)}}; }, term, term);
+        [&](int) { return Rec2<T>{}; }, term, [&](const Rec2<T>&) {});
     sum = group_sum<kCoop>(sum);
     if (c != 0) continue;
+    const V4<T> B = a1b[i];
+    sum += va.x * B.x + va.y * B.y + va.z * B.z;
     const T rp = rest_psi_of(k, rdens, i);
     T psi = (psi_rho[i] - rp) + sum * dt;
     if (psi < T(0)) psi = T(0);
@@ -416,9 +430,10 @@ __global__ void __launch_bounds__(kBlock)
                  const int* __restrict__ bstart, const int* __restrict__ nbr_f,
                  const int* __restrict__ nbr_b, const int* __restrict__ nbr_nf,
                  const int* __restrict__ nbr_nb, T* __restrict__ psi_adv, T* __restrict__ kappa,
-                 V4<T>* __restrict__ krec, V4<T>* __restrict__ tape_v, T* __restrict__ tape_k,
+                 V4<T>* __restrict__ krec, const V4<T>* __restrict__ a1b,
+                 V4<T>* __restrict__ tape_v, T* __restrict__ tape_k,
                  double* __restrict__ partial, TsphControl* ctrl) {
-  k_solver_psi_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, n, cap, kk, fpos, vadv, psi_rho, rdens, alpha, fstart, bpos, bstart, nbr_f, nbr_b, nbr_nf, nbr_nb, psi_adv, kappa, krec, tape_v, tape_k, partial, ctrl);
+  k_solver_psi_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, n, cap, kk, fpos, vadv, psi_rho, rdens, alpha, fstart, bpos, bstart, nbr_f, nbr_b, nbr_nf, nbr_nb, psi_adv, kappa, krec, a1b, tape_v, tape_k, partial, ctrl);
 }
 
 // Jacobi half-step U: IPPE_update_vel_adv (f,f)(f,b) (sph.py:229-244), with
@@ -844,13 +859,13 @@ __global__ void __launch_bounds__(kBlock)
 // ---------------------------------------------------------------------------
 template <class T>
 static int pass_a(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl* ctrl,
-                  bool with_bound, hipStream_t st) {
+                  bool with_bound, const WsLayout& L, hipStream_t st) {
   KP<T> k = make_kp<T>(prm);
   hipLaunchKernelGGL(k_step_reset, dim3(1), dim3(64), 0, st, ctrl);
   if (with_bound && f->n > 0 && b->n > 0) {
     const int nbf = blocks_for(f->n);
     hipLaunchKernelGGL(k_pass_a_both<T>, dim3(nbf + blocks_for(b->n)), dim3(kBlock), 0, st, k, *f,
-                       *b, nbf, ctrl);
+                       *b, nbf, (V4<T>*)L.a1b, ctrl);
     TSPH_LAUNCH_CHECK("density_alpha");
     return 0;
   }
@@ -862,7 +877,7 @@ static int pass_a(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphContr
                        (T*)f->sph_compression, (T*)f->sph_density, (V4<T>*)f->alpha_1,
                        (T*)f->alpha_2, (T*)f->alpha, (T*)f->vol, (const V4<T>*)f->vel,
                        (V4<T>*)f->vel_adv, f->nbr_f, f->nbr_b, f->nbr_nf, f->nbr_nb, f->flag,
-                       ctrl);
+                       (V4<T>*)L.a1b, ctrl);
   if (with_bound && b->n > 0)
     hipLaunchKernelGGL(k_pass_a_bound<T>, dim3(blocks_for(b->n)), dim3(kBlock), 0, st, k, b->n,
                        (const V4<T>*)b->pos, (const T*)b->mass, (const T*)b->rest_volume,
@@ -880,7 +895,7 @@ int density_alpha(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphContr
   WsLayout L;
   TSPH_REQUIRE(carve_ws(ws, wsb, f->capacity, prm.node_num, &L), TSPH_ECAPACITY,
                "tsph_density_alpha: workspace too small");
-  return pass_a<T>(prm, f, b, ctrl, true, st);
+  return pass_a<T>(prm, f, b, ctrl, true, L, st);
 }
 
 // The fluid half of pass A alone: rebuilds the neighbour lists and the
@@ -889,7 +904,10 @@ int density_alpha(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphContr
 template <class T>
 int relist(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl* ctrl, void* ws,
            size_t wsb, hipStream_t st) {
-  return pass_a<T>(prm, f, b, ctrl, false, st);
+  WsLayout L;
+  TSPH_REQUIRE(carve_ws(ws, wsb, f->capacity, prm.node_num, &L), TSPH_ECAPACITY,
+               "tsph_relist: workspace too small");
+  return pass_a<T>(prm, f, b, ctrl, false, L, st);
 }
 
 template <class T>
@@ -917,8 +935,8 @@ int solver_half(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl
                        (const V4<T>*)f->pos, (const V4<T>*)f->vel_adv, psi,
                        (const T*)f->rest_density, (const T*)f->alpha, f->cell_start,
                        (const V4<T>*)b->pos, b->cell_start, f->nbr_f, f->nbr_b, f->nbr_nf,
-                       f->nbr_nb, (T*)f->psi_adv, (T*)f->kappa, (V4<T>*)L.krec, (V4<T>*)tape_v,
-                       (T*)tape_k, L.partial, ctrl);
+                       f->nbr_nb, (T*)f->psi_adv, (T*)f->kappa, (V4<T>*)L.krec,
+                       (const V4<T>*)L.a1b, (V4<T>*)tape_v, (T*)tape_k, L.partial, ctrl);
   if (half & 2)
     hipLaunchKernelGGL(k_solver_update<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
                        kk, (const V4<T>*)f->pos, (const T*)f->mass, (const T*)f->kappa,
@@ -1166,7 +1184,7 @@ __global__ void __launch_bounds__(kBlock) kb_pass_a_fluid(const char* __restrict
           (const T*)b.rest_volume, b.cell_start, (T*)f.W, (T*)f.sph_compression,
           (T*)f.sph_density, (V4<T>*)f.alpha_1, (T*)f.alpha_2, (T*)f.alpha, (T*)f.vol,
           (const V4<T>*)f.vel, (V4<T>*)f.vel_adv, f.nbr_f, f.nbr_b, f.nbr_nf, f.nbr_nb, f.flag,
-          e.ctrl);
+          (V4<T>*)e.L.a1b, e.ctrl);
     };
     if (e.k.fastq)
       run(std::true_type{});
@@ -1202,7 +1220,8 @@ __global__ void __launch_bounds__(kBlock) kb_solver_psi(const char* __restrict__
                          (const V4<T>*)f.vel_adv, psi_of(e), (const T*)f.rest_density,
                          (const T*)f.alpha, f.cell_start, (const V4<T>*)e.b.pos, e.b.cell_start,
                          f.nbr_f, f.nbr_b, f.nbr_nf, f.nbr_nb, (T*)f.psi_adv, (T*)f.kappa,
-                         (V4<T>*)e.L.krec, (V4<T>*)nullptr, (T*)nullptr, e.L.partial, e.ctrl);
+                         (V4<T>*)e.L.krec, (const V4<T>*)e.L.a1b, (V4<T>*)nullptr, (T*)nullptr,
+                         e.L.partial, e.ctrl);
   });
 }
 
