@@ -326,10 +326,10 @@ int tsph_solver_psi_taped(const TsphParams* p, TsphObject* fluid, TsphObject* bo
                           TsphControl* ctrl, int32_t k, void* tape_vel_adv, void* tape_kappa,
                           void* workspace, size_t ws_bytes, void* stream);
 
-/* The fluid half of tsph_density_alpha alone: neighbour lists, density and
- * alpha of the fluid recomputed (bit-identically) from recorded sorted
- * positions, masses and cell table, so that per-step lists are recomputed
- * instead of stored. */
+/* The fluid's neighbour lists and counts of tsph_density_alpha alone,
+ * rebuilt (bit-identically) from recorded sorted positions and cell table,
+ * so that per-step lists are recomputed instead of stored.  Densities and
+ * alpha factors are not recomputed: the adjoint keeps them in its tapes. */
 int tsph_relist(const TsphParams* p, TsphObject* fluid, TsphObject* bound, TsphControl* ctrl,
                 void* workspace, size_t ws_bytes, void* stream);
 

@@ -22,10 +22,10 @@ by step.  A replayed step records exactly the tape of the forward step (the
 same kernels on the same state, with the forward's iteration counts), so
 the gradients do not depend on the budget; the budget only trades HBM for
 the replay work.  A tape holds the sorted positions, masses, rest
-densities, pids and cell table, vel_adv and kappa of every Jacobi iteration,
-and the advection's velocity and normals.  Neighbour lists, densities and
-alpha factors are not stored: `tsph_relist` recomputes them bit-identically
-from the recorded positions.
+densities, pids and cell table, the densities, alpha factors, volumes and
+alpha_1 of pass A, vel_adv and kappa of every Jacobi iteration, and the
+advection's velocity and normals.  Neighbour lists are not stored:
+`tsph_relist` recomputes them bit-identically from the recorded positions.
 
     runner = DiffRunner(sim)
     x, v = rollout(runner, v0, mu, gamma, steps=200, every=20)
@@ -42,14 +42,15 @@ from . import _lib
 class StepTape:
     """Everything the reverse of one step reads that is not recomputed."""
 
-    __slots__ = ("pos", "mass", "rest_density", "pid", "cell_start", "vel", "normal", "iters",
-                 "calls")
+    __slots__ = ("pos", "mass", "rest_density", "pid", "cell_start", "density", "alpha", "vol",
+                 "alpha_1", "vel", "normal", "iters", "calls")
 
     def __init__(self):
         self.iters = ([], [])  # per solve: [(vel_adv entering it, kappa), ...]
 
     def nbytes(self):
-        ts = [self.pos, self.mass, self.rest_density, self.pid, self.cell_start, self.vel, self.normal]
+        ts = [self.pos, self.mass, self.rest_density, self.pid, self.cell_start, self.density, self.alpha,
+              self.vol, self.alpha_1, self.vel, self.normal]
         ts += [t for s in self.iters for pair in s for t in pair]
         return sum(t.numel() * t.element_size() for t in ts)
 
@@ -157,6 +158,11 @@ class DiffRunner:
             tp.cell_start = f.t["cell_start"].clone()
         fs, bs = ctypes.byref(f.struct()), ctypes.byref(b.struct())
         sim._call("density_alpha", sim.lib.tsph_density_alpha, P, fs, bs, sim.ctrl_ptr(), ws, wsb, stp)
+        if tape:
+            tp.density = f.t["sph_density"][:n].clone()
+            tp.alpha = f.t["alpha"][:n].clone()
+            tp.vol = f.t["vol"][:n].clone()
+            tp.alpha_1 = f.t["alpha_1"][:n].clone()
         d_it = sim._solve(0, tape=tp.iters[0] if tape else None, force=force[0])
         sim._call("advect", sim.lib.tsph_advect, P, fs, bs, sim.ctrl_ptr(), ws, wsb, stp)
         if tape:
@@ -186,6 +192,10 @@ class DiffRunner:
         f.t["mass"][:n].copy_(tp.mass)
         f.t["rest_density"][:n].copy_(tp.rest_density)
         f.t["cell_start"].copy_(tp.cell_start)
+        f.t["sph_density"][:n].copy_(tp.density)
+        f.t["alpha"][:n].copy_(tp.alpha)
+        f.t["vol"][:n].copy_(tp.vol)
+        f.t["alpha_1"][:n].copy_(tp.alpha_1)
         fs, bs = ctypes.byref(f.struct()), ctypes.byref(b.struct())
         A = ctypes.byref(self.adj)
         vp = ctypes.c_void_p
