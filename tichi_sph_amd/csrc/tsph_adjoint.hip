@@ -298,17 +298,19 @@ __global__ void __launch_bounds__(kBlock)
         T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
         T r2 = dx * dx + dy * dy + dz * dz;
         if (!(r2 > T(0)) || !(d.vfj > T(0.99))) return;
-        T r = s_sqrt(r2);
+        const T rinv = f_rsqrt(r2);
+        T r = r2 * rinv;
         T C, Cp;
         coh_d1(k, r, &C, &Cp);
-        T cr = C / r;
-        T cq = (Cp - cr) / r2;
+        T cr = C * rinv;
+        T cq = (Cp - cr) * (rinv * rinv);
         const V4<T>& nj = d.nj;
         const V4<T>& uj = d.uj;
         T alj_x = dt * uj.x, alj_y = dt * uj.y, alj_z = dt * uj.z;
         T rhoj = d.rhoj, r0j = d.r0j, mj = d.mj;
         T den = rhoi + rhoj;
-        T fij = T(2) * r0i / den, fji = T(2) * r0j / den;
+        const T rden = f_rcp(den);
+        T fij = T(2) * r0i * rden, fji = T(2) * r0j * rden;
         T dnx = ni.x - nj.x, dny = ni.y - nj.y, dnz = ni.z - nj.z;
         // gamma-free force of the pair on l
         T tx = -mj * cr * dx + dnx, ty = -mj * cr * dy + dny, tz = -mj * cr * dz + dnz;
@@ -327,7 +329,7 @@ __global__ void __launch_bounds__(kBlock)
         T sl = ali_x * tx + ali_y * ty + ali_z * tz;
         T sj = alj_x * (mi * cr * dx - dnx) + alj_y * (mi * cr * dy - dny) +
                alj_z * (mi * cr * dz - dnz);
-        rb += -gam * (fij * sl + fji * sj) / den;
+        rb += -gam * (fij * sl + fji * sj) * rden;
       });
     }
     nbar[i] = v4(nbx, nby, nbz);

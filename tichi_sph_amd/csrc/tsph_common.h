@@ -234,6 +234,16 @@ __device__ __forceinline__ float f_rcp<float>(float x) {
   return __builtin_amdgcn_rcpf(x);
 }
 
+// The cohesion kernel C (sph_util.py:4-13) of q = r / h, in fast arithmetic.
+template <class T>
+__device__ __forceinline__ T f_coh(const KP<T>& k, T q) {
+  const T u = T(1) - q;
+  const T uq = u * q;
+  const T uq3 = uq * uq * uq;
+  const T c = q <= T(0.5) ? T(2) * uq3 - T(1.0 / 64) : (q < T(1) ? uq3 : T(0));
+  return c * k.c_coh;
+}
+
 // g = W_grad(r)/r so that grad W_ij = g * x_ij, for 0 < r2 < r2max.
 // Returns r through *r_out.
 template <class T>

@@ -616,7 +616,6 @@ __device__ __forceinline__ void k_surface_tension_body(Blk blk, KP<T> k, int n, 
                       const int* __restrict__ fstart, const V4<T>* __restrict__ normal,
                       const V4<T>* __restrict__ vel, V4<T>* __restrict__ acce_adv,
                       V4<T>* __restrict__ vadv, const TsphControl* ctrl) {
-  TSPH_STRICT
   int i = xcd_block(blk.b, blk.nb) * kBlock + threadIdx.x;
   if (i >= n) return;
   const T dt = (T)ctrl->dt;
@@ -636,22 +635,30 @@ __device__ __forceinline__ void k_surface_tension_body(Blk blk, KP<T> k, int n, 
       return Cand{fpos[j], normal[j], fmass[j], psi_rho[j], vf[(long)j * k.P]};
     };
     walk_cells_prefetch<kStU>(k, fstart, c, load, [&](int, const Cand& d) {
+      // every candidate of the 27 cells: reciprocal square root and
+      // reciprocal instead of the IEEE square root and divisions (the pair
+      // terms of this walk are not part of the bit-exact sums)
       const V4<T>& q = d.q;
-      T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-      T r = s_sqrt(s_r2(dx, dy, dz));
-      if (r > T(0) && d.vfj > T(0.99)) {
-        T cs = -k.gamma * d.mj * s_C(k, r);
+      const T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+      const T r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 > T(0) && d.vfj > T(0.99)) {
+        const T rinv = f_rsqrt(r2);
+        const T cs = -k.gamma * d.mj * f_coh(k, r2 * rinv * k.inv_h);
         const V4<T>& nj = d.nj;
-        T f = T(2) * rp / (psi_i + d.psij);
-        a.x = a.x + f * (cs * dx / r + k.gamma * (ni.x - nj.x));
-        a.y = a.y + f * (cs * dy / r + k.gamma * (ni.y - nj.y));
-        a.z = a.z + f * (cs * dz / r + k.gamma * (ni.z - nj.z));
+        const T f = T(2) * rp * f_rcp(psi_i + d.psij);
+        const T csr = cs * rinv;
+        a.x += f * (csr * dx + k.gamma * (ni.x - nj.x));
+        a.y += f * (csr * dy + k.gamma * (ni.y - nj.y));
+        a.z += f * (csr * dz + k.gamma * (ni.z - nj.z));
       }
     });
     acce_adv[i] = a;
   }
-  V4<T> v = vel[i];
-  vadv[i] = V4<T>{v.x + a.x * dt, v.y + a.y * dt, v.z + a.z * dt, T(0)};
+  {
+    TSPH_STRICT  // SPH_advection_update_vel_adv, sph.py:247-250
+    const V4<T> v = vel[i];
+    vadv[i] = V4<T>{v.x + a.x * dt, v.y + a.y * dt, v.z + a.z * dt, T(0)};
+  }
 }
 
 template <class T>
