@@ -240,6 +240,9 @@ class Oracle:
                 pos_l.append(x)
                 vel_l.append(np.tile(np.asarray(p["vel"], dtype=np.float32), (len(x), 1)))
                 vf_l.append(np.tile(np.asarray(p["volume_frac"], dtype=np.float32), (len(x), 1)))
+            if not pos_l:  # an object list with no particles (main.py:62-79 adds nothing)
+                self.set_part_num(obj, 0)
+                continue
             pos = np.concatenate(pos_l).astype(np.float32)
             n = len(pos)
             self.set_part_num(obj, n)

@@ -980,8 +980,10 @@ int solver_half(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl
   WsLayout L;
   TSPH_REQUIRE(carve_ws(ws, wsb, f->capacity, prm.node_num, &L), TSPH_ECAPACITY,
                "tsph_solver_iteration: workspace too small");
-  TSPH_REQUIRE(f->n > 0, TSPH_EINVAL, "tsph_solver_iteration: no fluid particles");
-  int nb = blocks_for(f->n);
+  // With no fluid particles one block still runs the stop rule: the mean of
+  // an empty sum is 0/0 = NaN, as in the reference (sph.py:221-226), so the
+  // solve runs its minimum iteration count.
+  int nb = blocks_for(f->n > 0 ? f->n : 1);
   const T* psi = (const T*)(prm.solver_type == TSPH_VFSPH ? f->sph_compression : f->sph_density);
   if (half & 1)
     hipLaunchKernelGGL(k_solver_psi<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity, kk,

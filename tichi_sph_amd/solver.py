@@ -451,9 +451,10 @@ class Simulation:
         return it
 
     def sph_step(self, stats=False):
-        """One step of main.py:175-256 (preceded by cfl_condition, main.py:348)."""
-        if self.fluid.part_num == 0:
-            raise Exception("sph_step: no fluid particles")
+        """One step of main.py:175-256 (preceded by cfl_condition, main.py:348).
+        With no fluid particles it still sorts and updates the boundary, and
+        each solve runs its minimum iteration count (the reference's empty
+        mean is NaN)."""
         P = ctypes.byref(self.params)
         ws, wsb = self._ws()
         st = self.stream_ptr()
