@@ -336,14 +336,34 @@ __device__ __forceinline__ double block_min(double v, double* lds) {
 
 // Deterministic fixed-order sum of `n` partials by one block (result in
 // thread 0).
+// Block reductions of n partials: each thread loads eight at a time, all in
+// flight together (one block over a few thousand partials is bound by the
+// load latency), and adds them in index order, so the result is fixed.
+constexpr int kPartU = 8;
 __device__ __forceinline__ double sum_partials(const double* part, int n, double* lds) {
   double v = 0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) v += __builtin_nontemporal_load(part + i);
+  const int s = blockDim.x;
+  for (int base = threadIdx.x; base < n; base += kPartU * s) {
+    double a[kPartU];
+#pragma unroll
+    for (int u = 0; u < kPartU; ++u)
+      a[u] = base + u * s < n ? __builtin_nontemporal_load(part + base + u * s) : 0.0;
+#pragma unroll
+    for (int u = 0; u < kPartU; ++u) v += a[u];
+  }
   return block_sum(v, lds);
 }
 __device__ __forceinline__ double min_partials(const double* part, int n, double* lds, double init) {
   double v = init;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) v = fmin(v, __builtin_nontemporal_load(part + i));
+  const int s = blockDim.x;
+  for (int base = threadIdx.x; base < n; base += kPartU * s) {
+    double a[kPartU];
+#pragma unroll
+    for (int u = 0; u < kPartU; ++u)
+      a[u] = base + u * s < n ? __builtin_nontemporal_load(part + base + u * s) : init;
+#pragma unroll
+    for (int u = 0; u < kPartU; ++u) v = fmin(v, a[u]);
+  }
   return block_min(v, lds);
 }
 
