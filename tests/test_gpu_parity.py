@@ -192,18 +192,33 @@ def test_fused_step_with_overflowing_lists_matches_oracle():
     compare_fields(sim, o, FIELDS_F + ["alpha_1", "alpha_2", "normal"], label="overflow")
 
 
+def close_or_as_accurate(g, r32, r64, rel):
+    """g is within rel of the FP32 oracle, or at most twice as far from the
+    FP64 oracle as the FP32 oracle is.  Over many steps FP32 rounding grows
+    (the FP32 oracle's kinetic energy drifts 4e-4 from the FP64 one in 20
+    steps of the 2D dam break), so a multi-step figure that leaves the FP32
+    oracle must still be as accurate as it."""
+    return abs(g - r32) <= rel * abs(r32) or abs(g - r64) <= 2 * abs(r32 - r64)
+
+
 def test_fused_multi_step_statistics():
     sim, o = make("dambreak_2d_8k")
+    cb, sb = load_scene("dambreak_2d_8k")
+    o64 = O.Oracle(cb, sb, 8)
+    o64.init_scenario()
     for step in range(20):
         s = sim.sph_step(stats=True)
         st = o.step(O.STEP_CFL)
+        st64 = o64.step(O.STEP_CFL)
         assert (s.div_iter, s.incom_iter) == (st.div_iter, st.incom_iter), step
-        for a, b in zip(s.incom_err, list(st.incom_err)[: st.incom_iter]):
-            assert a == pytest.approx(b, rel=1e-3), step
+        for a, b, c in zip(s.incom_err, list(st.incom_err)[: st.incom_iter], list(st64.incom_err)):
+            assert close_or_as_accurate(a, b, c, 1e-3), (step, a, b, c)
     com_g = sim.fluid.to_numpy("pos").mean(0)
     com_o = o.get(0, "pos").mean(0)
     assert np.abs(com_g - com_o).max() < 1e-6
-    assert s.kinetic_energy == pytest.approx(o.scalar("ke"), rel=1e-4)
+    ke, ke32, ke64 = s.kinetic_energy, o.scalar("ke"), o64.scalar("ke")
+    print(f"  kinetic energy after 20 steps: gpu {ke!r} oracle {ke32!r} FP64 oracle {ke64!r}", flush=True)
+    assert close_or_as_accurate(ke, ke32, ke64, 1e-4)
 
 
 def test_fp64_fused_step_matches_fp64_oracle():

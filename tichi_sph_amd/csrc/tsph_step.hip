@@ -26,19 +26,22 @@ struct PassAAcc {
   __device__ __forceinline__ bool add(const KP<T>& k, const V4<T>& p, const V4<T>& q, T mj, T vj,
                                       bool fluid) {
     T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-    T r2;
+    T r2, r, qh;
     {
       TSPH_STRICT
       r2 = s_r2(dx, dy, dz);
-      T r = pa_sqrt<FQ>(r2);
-      T Wr = s_W_q_sel(k, pa_quot<FQ>(k, r));
+      r = pa_sqrt<FQ>(r2);
+      qh = pa_quot<FQ>(k, r);
+      T Wr = s_W_q_sel(k, qh);
       w = w + Wr;
       cp = cp + Wr * vj;
       dn = dn + Wr * mj;
       if (!(r > T(0))) return false;
     }
-    T rr;
-    T g = f_grad(k, r2, &rr);
+    // W'(q) / r from the r and q just formed: 6 sig/h^2 (3q - 2) for q <= 1/2
+    // (no division), -6 sig/h (1 - q)^2 / r above
+    const T u = T(1) - qh;
+    const T g = qh <= T(0.5) ? k.wg_a * (T(3) * qh - T(2)) : -k.wg_b * (u * u) * f_rcp(r);
     T Xj = q.w;
     T xg = Xj * g;
     ax += xg * dx;
@@ -118,18 +121,27 @@ __device__ __forceinline__ void k_pass_a_fluid_body(Blk blk, KP<T> k, int n, int
   node_encode(k, p.x, p.y, p.z, c);
   PassAAcc<T> acc;
   int nf = 0, nbb = 0;
+  // pos.w is the solver alias X (mass for DFSPH, rest volume for VFSPH), so
+  // only the other of the two fields is loaded
+  const bool xm = k.solver != TSPH_VFSPH;
+  const T* __restrict__ fo = xm ? frvol : fmass;
+  const T* __restrict__ bo = xm ? brvol : bmass;
   walk_cells_batched<kPassAU>(k, fstart, c, fpos, [&](int j, const V4<T>& q) {
-    if (s_r2(p.x - q.x, p.y - q.y, p.z - q.z) <= k.r2max &&
-        acc.template add<FQ>(k, p, q, fmass[j], frvol[j], true)) {
-      if (nf < capf) nbr_f[(long)nf * cap + i] = j;
-      ++nf;
+    if (s_r2(p.x - q.x, p.y - q.y, p.z - q.z) <= k.r2max) {
+      const T o = fo[j];
+      if (acc.template add<FQ>(k, p, q, xm ? q.w : o, xm ? o : q.w, true)) {
+        if (nf < capf) nbr_f[(long)nf * cap + i] = j;
+        ++nf;
+      }
     }
   });
   walk_cells_batched<kPassAU>(k, bstart, c, bpos, [&](int j, const V4<T>& q) {
-    if (s_r2(p.x - q.x, p.y - q.y, p.z - q.z) <= k.r2max &&
-        acc.template add<FQ>(k, p, q, bmass[j], brvol[j], false)) {
-      if (nbb < capb) nbr_b[(long)nbb * cap + i] = j;
-      ++nbb;
+    if (s_r2(p.x - q.x, p.y - q.y, p.z - q.z) <= k.r2max) {
+      const T o = bo[j];
+      if (acc.template add<FQ>(k, p, q, xm ? q.w : o, xm ? o : q.w, false)) {
+        if (nbb < capb) nbr_b[(long)nbb * cap + i] = j;
+        ++nbb;
+      }
     }
   });
   a1b[i] = V4<T>{acc.bx, acc.by, acc.bz, T(0)};
