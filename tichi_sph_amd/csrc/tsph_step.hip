@@ -22,7 +22,8 @@ template <class T>
 struct PassAAcc {
   T w = 0, cp = 0, dn = 0, ax = 0, ay = 0, az = 0, a2 = 0;
   T bx = 0, by = 0, bz = 0;  // the boundary neighbours' share of alpha_1
-  template <bool FQ>
+  // XM: the solver alias X is the mass (DFSPH), so X_j = m_j
+  template <bool FQ, bool XM>
   __device__ __forceinline__ bool add(const KP<T>& k, const V4<T>& p, const V4<T>& q, T mj, T vj,
                                       bool fluid) {
     T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
@@ -48,7 +49,9 @@ struct PassAAcc {
     ay += xg * dy;
     az += xg * dz;
     if (fluid) {
-      a2 += (g * g) * r2 * (Xj * Xj) * f_rcp(mj);
+      // W'^2 X_j^2 / m_j; for DFSPH X_j is m_j and the quotient is X_j
+      const T x2m = XM ? Xj : (Xj * Xj) * f_rcp(mj);
+      a2 += (g * g) * r2 * x2m;
     } else {
       bx += xg * dx;
       by += xg * dy;
@@ -100,7 +103,7 @@ __device__ __forceinline__ void pass_a_epilogue(const KP<T>& k, int i, const V4<
   nbr_nb[i] = nbb;
 }
 
-template <class T, bool FQ>
+template <class T, bool FQ, bool XM>
 __device__ __forceinline__ void k_pass_a_fluid_body(Blk blk, KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
                    const T* __restrict__ fmass, const T* __restrict__ frvol,
                    const int* __restrict__ fstart, int nb, const V4<T>* __restrict__ bpos,
@@ -123,13 +126,12 @@ __device__ __forceinline__ void k_pass_a_fluid_body(Blk blk, KP<T> k, int n, int
   int nf = 0, nbb = 0;
   // pos.w is the solver alias X (mass for DFSPH, rest volume for VFSPH), so
   // only the other of the two fields is loaded
-  const bool xm = k.solver != TSPH_VFSPH;
-  const T* __restrict__ fo = xm ? frvol : fmass;
-  const T* __restrict__ bo = xm ? brvol : bmass;
+  const T* __restrict__ fo = XM ? frvol : fmass;
+  const T* __restrict__ bo = XM ? brvol : bmass;
   walk_cells_batched<kPassAU>(k, fstart, c, fpos, [&](int j, const V4<T>& q) {
     if (s_r2(p.x - q.x, p.y - q.y, p.z - q.z) <= k.r2max) {
       const T o = fo[j];
-      if (acc.template add<FQ>(k, p, q, xm ? q.w : o, xm ? o : q.w, true)) {
+      if (acc.template add<FQ, XM>(k, p, q, XM ? q.w : o, XM ? o : q.w, true)) {
         if (nf < capf) nbr_f[(long)nf * cap + i] = j;
         ++nf;
       }
@@ -138,7 +140,7 @@ __device__ __forceinline__ void k_pass_a_fluid_body(Blk blk, KP<T> k, int n, int
   walk_cells_batched<kPassAU>(k, bstart, c, bpos, [&](int j, const V4<T>& q) {
     if (s_r2(p.x - q.x, p.y - q.y, p.z - q.z) <= k.r2max) {
       const T o = bo[j];
-      if (acc.template add<FQ>(k, p, q, xm ? q.w : o, xm ? o : q.w, false)) {
+      if (acc.template add<FQ, XM>(k, p, q, XM ? q.w : o, XM ? o : q.w, false)) {
         if (nbb < capb) nbr_b[(long)nbb * cap + i] = j;
         ++nbb;
       }
@@ -147,6 +149,22 @@ __device__ __forceinline__ void k_pass_a_fluid_body(Blk blk, KP<T> k, int n, int
   a1b[i] = V4<T>{acc.bx, acc.by, acc.bz, T(0)};
   pass_a_epilogue(k, i, p, mi, acc, nf, nbb, W, comp, dens, alpha_1, alpha_2, alpha, vol, vel,
                   vel_adv, nbr_nf, nbr_nb, ctrl);
+}
+
+// Runs f(fq, xm) with the fluid body's compile-time variant for these
+// constants: the verified fast quotient, and whether X is the mass.
+template <class T, class F>
+__device__ __forceinline__ void pass_a_fluid_variant(const KP<T>& k, F&& f) {
+  using Y = std::true_type;
+  using N = std::false_type;
+  const bool xm = k.solver != TSPH_VFSPH;
+  if (k.fastq) {
+    if (xm) f(Y{}, Y{});
+    else f(Y{}, N{});
+  } else {
+    if (xm) f(N{}, Y{});
+    else f(N{}, N{});
+  }
 }
 
 template <class T>
@@ -162,10 +180,9 @@ __global__ void __launch_bounds__(kBlock)
                    int* __restrict__ nbr_nf, int* __restrict__ nbr_nb, int* __restrict__ flag,
                    V4<T>* __restrict__ a1b, TsphControl* ctrl) {
   const Blk blk{(int)blockIdx.x, (int)gridDim.x};
-  if (k.fastq)
-    k_pass_a_fluid_body<T, true>(blk, k, n, cap, fpos, fmass, frvol, fstart, nb, bpos, bmass, brvol, bstart, W, comp, dens, alpha_1, alpha_2, alpha, vol, vel, vel_adv, nbr_f, nbr_b, nbr_nf, nbr_nb, flag, a1b, ctrl);
-  else
-    k_pass_a_fluid_body<T, false>(blk, k, n, cap, fpos, fmass, frvol, fstart, nb, bpos, bmass, brvol, bstart, W, comp, dens, alpha_1, alpha_2, alpha, vol, vel, vel_adv, nbr_f, nbr_b, nbr_nf, nbr_nb, flag, a1b, ctrl);
+  pass_a_fluid_variant(k, [&](auto fq, auto xm) {
+    k_pass_a_fluid_body<T, decltype(fq)::value, decltype(xm)::value>(blk, k, n, cap, fpos, fmass, frvol, fstart, nb, bpos, bmass, brvol, bstart, W, comp, dens, alpha_1, alpha_2, alpha, vol, vel, vel_adv, nbr_f, nbr_b, nbr_nf, nbr_nb, flag, a1b, ctrl);
+  });
 }
 
 // Pass A, boundary particles: SPH_prepare_attr (b,b)(b,f), SPH_prepare_alpha_2
@@ -251,18 +268,20 @@ __global__ void k_step_reset(TsphControl* ctrl) { step_reset_body(ctrl); }
 // the rest the boundary body.  The two halves are independent (each writes
 // only its own object's fields), so the boundary blocks fill the tail of the
 // fluid's grid instead of waiting for it.
-template <class T, bool FQ>
+template <class T>
 __device__ __forceinline__ void pass_a_both_body(const KP<T>& k, const TsphObject& f,
                                                  const TsphObject& b, int nbf, V4<T>* a1b,
                                                  TsphControl* ctrl) {
   if ((int)blockIdx.x < nbf) {
-    k_pass_a_fluid_body<T, FQ>(Blk{(int)blockIdx.x, nbf}, k, f.n, f.capacity, (const V4<T>*)f.pos,
-                               (const T*)f.mass, (const T*)f.rest_volume, f.cell_start, b.n,
-                               (const V4<T>*)b.pos, (const T*)b.mass, (const T*)b.rest_volume,
-                               b.cell_start, (T*)f.W, (T*)f.sph_compression, (T*)f.sph_density,
-                               (V4<T>*)f.alpha_1, (T*)f.alpha_2, (T*)f.alpha, (T*)f.vol,
-                               (const V4<T>*)f.vel, (V4<T>*)f.vel_adv, f.nbr_f, f.nbr_b, f.nbr_nf,
-                               f.nbr_nb, f.flag, a1b, ctrl);
+    pass_a_fluid_variant(k, [&](auto fq, auto xm) {
+      k_pass_a_fluid_body<T, decltype(fq)::value, decltype(xm)::value>(
+          Blk{(int)blockIdx.x, nbf}, k, f.n, f.capacity, (const V4<T>*)f.pos, (const T*)f.mass,
+          (const T*)f.rest_volume, f.cell_start, b.n, (const V4<T>*)b.pos, (const T*)b.mass,
+          (const T*)b.rest_volume, b.cell_start, (T*)f.W, (T*)f.sph_compression,
+          (T*)f.sph_density, (V4<T>*)f.alpha_1, (T*)f.alpha_2, (T*)f.alpha, (T*)f.vol,
+          (const V4<T>*)f.vel, (V4<T>*)f.vel_adv, f.nbr_f, f.nbr_b, f.nbr_nf, f.nbr_nb, f.flag,
+          a1b, ctrl);
+    });
   } else {
     k_pass_a_bound_body<T, false>(Blk{(int)blockIdx.x - nbf, (int)gridDim.x - nbf}, k, b.n,
                                (const V4<T>*)b.pos, (const T*)b.mass, (const T*)b.rest_volume,
@@ -275,10 +294,7 @@ __device__ __forceinline__ void pass_a_both_body(const KP<T>& k, const TsphObjec
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_pass_a_both(KP<T> k, TsphObject f, TsphObject b, int nbf, V4<T>* a1b, TsphControl* ctrl) {
-  if (k.fastq)
-    pass_a_both_body<T, true>(k, f, b, nbf, a1b, ctrl);
-  else
-    pass_a_both_body<T, false>(k, f, b, nbf, a1b, ctrl);
+  pass_a_both_body<T>(k, f, b, nbf, a1b, ctrl);
 }
 
 // The check behind TSPH_FLAG_FAST_QUOTIENT: for every FP32 r^2 in
@@ -1244,19 +1260,15 @@ __global__ void __launch_bounds__(kBlock) kb_pass_a_fluid(const char* __restrict
   for_tiles<T, kTileFluid>(tab, ns, [&](const SceneEntry<T>& e, int, Blk blk) {
     const TsphObject& f = e.f;
     const TsphObject& b = e.b;
-    auto run = [&](auto fq) {
-      k_pass_a_fluid_body<T, decltype(fq)::value>(
+    pass_a_fluid_variant(e.k, [&](auto fq, auto xm) {
+      k_pass_a_fluid_body<T, decltype(fq)::value, decltype(xm)::value>(
           blk, e.k, f.n, f.capacity, (const V4<T>*)f.pos, (const T*)f.mass,
           (const T*)f.rest_volume, f.cell_start, b.n, (const V4<T>*)b.pos, (const T*)b.mass,
           (const T*)b.rest_volume, b.cell_start, (T*)f.W, (T*)f.sph_compression,
           (T*)f.sph_density, (V4<T>*)f.alpha_1, (T*)f.alpha_2, (T*)f.alpha, (T*)f.vol,
           (const V4<T>*)f.vel, (V4<T>*)f.vel_adv, f.nbr_f, f.nbr_b, f.nbr_nf, f.nbr_nb, f.flag,
           (V4<T>*)e.L.a1b, e.ctrl);
-    };
-    if (e.k.fastq)
-      run(std::true_type{});
-    else
-      run(std::false_type{});
+    });
   });
 }
 
