@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define TSPH_ABI_VERSION 2
+#define TSPH_ABI_VERSION 3
 #define TSPH_MAX_PHASE 8
 #define TSPH_MAX_ITER_LOG 256
 
@@ -75,7 +75,8 @@ typedef struct TsphParams {
  * sph_obj.py:8-93).  Names follow the reference attributes.
  */
 typedef struct TsphObject {
-  int32_t n, capacity, uid, pad;
+  int32_t n, capacity, uid;
+  int32_t flags; /* TSPH_OBJ_*: state the library keeps in the descriptor; zero it once */
   void *pos, *vel, *vel_adv, *acce_adv, *acce, *normal, *alpha_1;      /* T[cap][4] */
   void *mass, *rest_density, *rest_volume;                             /* T[cap] */
   void *volume_frac, *volume_frac_tmp;                                 /* T[cap][P] */
@@ -91,7 +92,18 @@ typedef struct TsphObject {
   /* back buffers of the persistent state, swapped by tsph_sort_object */
   void *pos_b, *vel_b, *mass_b, *rest_density_b, *rest_volume_b, *volume_frac_b;
   int32_t *pid_b;
+  /* T[cap][4], or null: the boundary's own-object (b,b) sums of W,
+   * sph_compression and sph_density in pass A's order.  The boundary is
+   * static, so they are kept from one step to the next (TSPH_OBJ_STATIC_SUMS)
+   * and each step only adds the (b,f) pairs, with the same result bit for
+   * bit.  A boundary with a null pointer is summed in full every step. */
+  void *static_sums;
 } TsphObject;
+
+/* TsphObject.flags: static_sums holds this object's current (b,b) sums.
+ * tsph_density_alpha sets it after summing them; tsph_sort_object clears it,
+ * and so must a caller that writes the object's pos, mass or rest_volume. */
+#define TSPH_OBJ_STATIC_SUMS 1
 
 /*
  * Device-resident control block: the reference's 0-D fields

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     assert set(decl) == set(_lib.EXPORTED)
     for name in decl:
         assert hasattr(lib, name), name
-    assert lib.tsph_abi_version() == _lib.ABI_VERSION == 2
+    assert lib.tsph_abi_version() == _lib.ABI_VERSION == 3
 
 
 def test_struct_layout_matches_header(tmp_path):
@@ -34,7 +34,8 @@ def test_struct_layout_matches_header(tmp_path):
     fields = {
         "TsphParams": ["dim", "list_cap_f", "grid_size", "cs", "list_cap_b", "lb", "phase_rest_density",
                        "part_size", "kernel_sig", "r2_max", "cfl_factor", "compression_threshold"],
-        "TsphObject": ["n", "pos", "pressure", "pid", "cell_structured", "cell_start", "nbr_nb", "pid_b"],
+        "TsphObject": ["n", "flags", "pos", "pressure", "pid", "cell_structured", "cell_start", "nbr_nb", "pid_b",
+                       "static_sums"],
         "TsphControl": ["dt", "it", "stop", "general_flag", "ke", "run_u", "err_div", "err_incom", "signal", "token",
                         "overflow_f", "counter2"],
         "TsphAdjoint": ["xbar", "vbar_out", "nbar", "a1bar", "kappa_next", "tape_normal", "grad"],

@@ -303,3 +303,34 @@ def test_full_size_sort_and_lists_properties():
     m = f.to_numpy("mass").astype(np.float64)
     v = f.to_numpy("vel").astype(np.float64)
     assert np.isfinite(v).all()
+
+
+def test_kept_boundary_sums_are_bit_identical():
+    """The boundary's (b,b) density sums are summed once and kept while the
+    boundary does not change; a run that sums them every step, and a write to
+    the boundary's masses (which makes the next step sum them again), give
+    the same bits."""
+    from tichi_sph_amd import _lib
+    a, _ = make("dambreak_3d_8k")
+    b, _ = make("dambreak_3d_8k")
+    b.bound.t["static_sums"] = None  # b sums the boundary in full every step
+    b.bound._refresh_struct()
+
+    def same():
+        for x, y in ((a.fluid, b.fluid), (a.bound, b.bound)):
+            for f in ("pos", "vel", "W", "sph_density", "sph_compression", "alpha", "alpha_2"):
+                assert np.array_equal(x.to_numpy(f), y.to_numpy(f)), f
+
+    for _ in range(3):
+        assert a.sph_step() == b.sph_step()
+    assert a.bound._struct.flags & _lib.TSPH_OBJ_STATIC_SUMS
+    assert not b.bound._struct.flags & _lib.TSPH_OBJ_STATIC_SUMS
+    same()
+    m = (a.bound.to_numpy("mass") * np.float32(1.01)).astype(np.float32)
+    for s in (a, b):
+        s.bound.from_numpy("mass", m)
+    assert not a.bound._struct.flags & _lib.TSPH_OBJ_STATIC_SUMS
+    for _ in range(2):
+        assert a.sph_step() == b.sph_step()
+    same()
+    assert np.array_equal(a.bound.to_numpy("mass"), m)

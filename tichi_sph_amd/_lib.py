@@ -9,8 +9,9 @@ import os
 
 TSPH_MAX_PHASE = 8
 TSPH_MAX_ITER_LOG = 256
-ABI_VERSION = 2  # TSPH_ABI_VERSION of include/tichi_sph.h
+ABI_VERSION = 3  # TSPH_ABI_VERSION of include/tichi_sph.h
 TSPH_FLAG_FAST_QUOTIENT = 1  # TsphParams.flags
+TSPH_OBJ_STATIC_SUMS = 1  # TsphObject.flags
 TSPH_DFSPH = 0
 TSPH_VFSPH = 1
 
@@ -49,11 +50,12 @@ _OBJ_PTRS = [
     "pid", "flag", "cell", "cell_structured", "in_cell", "cell_count", "cell_start",
     "nbr_f", "nbr_b", "nbr_nf", "nbr_nb",
     "pos_b", "vel_b", "mass_b", "rest_density_b", "rest_volume_b", "volume_frac_b", "pid_b",
+    "static_sums",
 ]
 
 
 class TsphObject(ctypes.Structure):
-    _fields_ = [("n", _i32), ("capacity", _i32), ("uid", _i32), ("pad", _i32)] + [
+    _fields_ = [("n", _i32), ("capacity", _i32), ("uid", _i32), ("flags", _i32)] + [
         (name, _vp) for name in _OBJ_PTRS
     ]
 

@@ -246,6 +246,7 @@ int sort_object(const TsphParams& prm, TsphObject* o, TsphControl* ctrl, int cfl
   WsLayout L;
   TSPH_REQUIRE(carve_ws(ws, wsb, o->capacity, prm.node_num, &L), TSPH_ECAPACITY,
                "tsph_sort_object: workspace too small (%zu bytes)", wsb);
+  o->flags &= ~TSPH_OBJ_STATIC_SUMS;  // new storage order: the kept sums no longer apply
   long len = (long)prm.node_num + 1;  // cells + trailing invalid bucket
   hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(len + 1)), dim3(kBlock), 0, st, o->cell_count,
                      len + 1, 0);

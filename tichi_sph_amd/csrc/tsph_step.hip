@@ -187,6 +187,10 @@ __global__ void __launch_bounds__(kBlock)
 
 // Pass A, boundary particles: SPH_prepare_attr (b,b)(b,f), SPH_prepare_alpha_2
 // (b,f), SPH_prepare_alpha (alpha_1 of the boundary is never accumulated).
+// The reference adds all (b,b) pairs before the (b,f) ones (main.py:189-190),
+// and the boundary never moves, so the (b,b) partial sums are kept in ss
+// (when given): with `kept` they are read instead of summed, otherwise summed
+// and stored.
 template <class T, bool FQ>
 __device__ __forceinline__ void k_pass_a_bound_body(Blk blk, KP<T> k, int nb, const V4<T>* __restrict__ bpos, const T* __restrict__ bmass,
                    const T* __restrict__ brvol, const int* __restrict__ bstart,
@@ -194,7 +198,7 @@ __device__ __forceinline__ void k_pass_a_bound_body(Blk blk, KP<T> k, int nb, co
                    const T* __restrict__ frvol, const int* __restrict__ fstart,
                    T* __restrict__ W, T* __restrict__ comp, T* __restrict__ dens,
                    V4<T>* __restrict__ alpha_1, T* __restrict__ alpha_2, T* __restrict__ alpha,
-                   T* __restrict__ vol) {
+                   T* __restrict__ vol, V4<T>* __restrict__ ss, bool kept) {
   TSPH_STRICT
   int i = xcd_block(blk.b, blk.nb) * kBlock + threadIdx.x;
   if (i >= nb) return;
@@ -202,15 +206,23 @@ __device__ __forceinline__ void k_pass_a_bound_body(Blk blk, KP<T> k, int nb, co
   int c[3];
   node_encode(k, p.x, p.y, p.z, c);
   T w = 0, cp = 0, dn = 0, a2 = 0;
-  walk_cells_batched<kPassAU>(k, bstart, c, bpos, [&](int j, const V4<T>& q) {
-    T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
-    if (r2 <= k.r2max) {
-      T Wr = s_W_q(k, pa_quot<FQ>(k, pa_sqrt<FQ>(r2)));
-      w = w + Wr;
-      cp = cp + Wr * brvol[j];
-      dn = dn + Wr * bmass[j];
-    }
-  });
+  if (ss != nullptr && kept) {
+    const V4<T> s0 = ss[i];
+    w = s0.x;
+    cp = s0.y;
+    dn = s0.z;
+  } else {
+    walk_cells_batched<kPassAU>(k, bstart, c, bpos, [&](int j, const V4<T>& q) {
+      T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
+      if (r2 <= k.r2max) {
+        T Wr = s_W_q(k, pa_quot<FQ>(k, pa_sqrt<FQ>(r2)));
+        w = w + Wr;
+        cp = cp + Wr * brvol[j];
+        dn = dn + Wr * bmass[j];
+      }
+    });
+    if (ss != nullptr) ss[i] = V4<T>{w, cp, dn, T(0)};
+  }
   walk_cells_batched<kPassAU>(k, fstart, c, fpos, [&](int j, const V4<T>& q) {
     T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
     if (r2 <= k.r2max) {
@@ -248,8 +260,8 @@ __global__ void __launch_bounds__(kBlock)
                    const T* __restrict__ frvol, const int* __restrict__ fstart,
                    T* __restrict__ W, T* __restrict__ comp, T* __restrict__ dens,
                    V4<T>* __restrict__ alpha_1, T* __restrict__ alpha_2, T* __restrict__ alpha,
-                   T* __restrict__ vol) {
-  k_pass_a_bound_body<T, false>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, nb, bpos, bmass, brvol, bstart, fpos, fmass, frvol, fstart, W, comp, dens, alpha_1, alpha_2, alpha, vol);
+                   T* __restrict__ vol, V4<T>* __restrict__ ss, int kept) {
+  k_pass_a_bound_body<T, false>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, nb, bpos, bmass, brvol, bstart, fpos, fmass, frvol, fstart, W, comp, dens, alpha_1, alpha_2, alpha, vol, ss, kept != 0);
 }
 
 // Per-step reset of the control block's step-scoped state: general_flag = 1
@@ -288,7 +300,8 @@ __device__ __forceinline__ void pass_a_both_body(const KP<T>& k, const TsphObjec
                                b.cell_start, (const V4<T>*)f.pos, (const T*)f.mass,
                                (const T*)f.rest_volume, f.cell_start, (T*)b.W,
                                (T*)b.sph_compression, (T*)b.sph_density, (V4<T>*)b.alpha_1,
-                               (T*)b.alpha_2, (T*)b.alpha, (T*)b.vol);
+                               (T*)b.alpha_2, (T*)b.alpha, (T*)b.vol, (V4<T>*)b.static_sums,
+                               (b.flags & TSPH_OBJ_STATIC_SUMS) != 0);
   }
 }
 template <class T>
@@ -902,6 +915,7 @@ static int pass_a(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphContr
     hipLaunchKernelGGL(k_pass_a_both<T>, dim3(nbf + blocks_for(b->n)), dim3(kBlock), 0, st, k, *f,
                        *b, nbf, (V4<T>*)L.a1b, ctrl);
     TSPH_LAUNCH_CHECK("density_alpha");
+    if (b->static_sums) b->flags |= TSPH_OBJ_STATIC_SUMS;  // stream order: later launches read them
     return 0;
   }
   if (f->n > 0)
@@ -919,8 +933,9 @@ static int pass_a(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphContr
                        b->cell_start, (const V4<T>*)f->pos, (const T*)f->mass,
                        (const T*)f->rest_volume, f->cell_start, (T*)b->W, (T*)b->sph_compression,
                        (T*)b->sph_density, (V4<T>*)b->alpha_1, (T*)b->alpha_2, (T*)b->alpha,
-                       (T*)b->vol);
+                       (T*)b->vol, (V4<T>*)b->static_sums, b->flags & TSPH_OBJ_STATIC_SUMS);
   TSPH_LAUNCH_CHECK("density_alpha");
+  if (with_bound && b->n > 0 && b->static_sums) b->flags |= TSPH_OBJ_STATIC_SUMS;
   return 0;
 }
 
@@ -1281,7 +1296,8 @@ __global__ void __launch_bounds__(kBlock) kb_pass_a_bound(const char* __restrict
         blk, e.k, b.n, (const V4<T>*)b.pos, (const T*)b.mass, (const T*)b.rest_volume,
         b.cell_start, (const V4<T>*)f.pos, (const T*)f.mass, (const T*)f.rest_volume,
         f.cell_start, (T*)b.W, (T*)b.sph_compression, (T*)b.sph_density, (V4<T>*)b.alpha_1,
-        (T*)b.alpha_2, (T*)b.alpha, (T*)b.vol);
+        (T*)b.alpha_2, (T*)b.alpha, (T*)b.vol, (V4<T>*)b.static_sums,
+        (b.flags & TSPH_OBJ_STATIC_SUMS) != 0);
   });
 }
 

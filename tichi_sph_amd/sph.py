@@ -13,6 +13,7 @@ from . import _lib
 
 
 def _run(sim, name, obj, nobj=None):
+    obj.invalidate_static_sums()  # a reference kernel may write obj's positions or masses
     P = ctypes.byref(sim.params)
     ws = ctypes.c_void_p(sim.ws.data_ptr())
     npc, nps, pin, uin = sim.ngrid.ptrs()

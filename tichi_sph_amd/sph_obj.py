@@ -72,6 +72,8 @@ class Fluid:
             self.t[name + "_b"] = z(cap)
         self.t["volume_frac_b"] = z(cap, P)
         self.t["pid_b"] = zi(cap)
+        # the boundary keeps its own-object density sums between steps
+        self.t["static_sums"] = z(cap, 4) if self.uid == 1 else None
         self._struct = _lib.TsphObject()
         self._refresh_struct()
         self.cell_table_ok = False  # cell_start matches the storage (set by the sort)
@@ -83,7 +85,8 @@ class Fluid:
         s.capacity = max(self.max_part_num, 1)
         s.uid = self.uid
         for name in _lib._OBJ_PTRS:
-            setattr(s, name, self.t[name].data_ptr())
+            t = self.t[name]
+            setattr(s, name, t.data_ptr() if t is not None else None)
 
     def struct(self):
         self._struct.n = self.part_num
@@ -136,8 +139,14 @@ class Fluid:
         out[pid] = a
         return out
 
+    def invalidate_static_sums(self):
+        """The kept own-object density sums no longer match the particles:
+        the next pass A sums them again."""
+        self._struct.flags &= ~_lib.TSPH_OBJ_STATIC_SUMS
+
     def from_numpy(self, name, arr):
         """Write field `name` from an array in reference particle order."""
+        self.invalidate_static_sums()
         pid = self.t["pid"][: self.part_num].cpu().numpy()
         a = np.asarray(arr)[pid]
         dst = self._storage(name)
