@@ -334,3 +334,56 @@ def test_kept_boundary_sums_are_bit_identical():
         assert a.sph_step() == b.sph_step()
     same()
     assert np.array_equal(a.bound.to_numpy("mass"), m)
+
+
+def test_two_phase_2m_invariants():
+    """The 2M-particle two-phase VFSPH mixture (phase A, density 1000, below
+    phase B, density 500; volume fractions 1/0 with a seeded +-0.02
+    perturbation, renormalised; surface tension 0.07, drift-diffusion 1e-4)
+    stepped with FBM rounds: invariants that hold at any size.  Each
+    particle's fractions sum to 1 and stay non-negative, each phase's volume
+    sum_i V_i phi_ik is conserved (the fluxes are antisymmetric per pair),
+    rest density and mass follow sph.py:282-286 bit for bit, and the state
+    stays finite."""
+    import os
+    import time
+    from tichi_sph_amd import load_simulation
+    from conftest import SCEN
+    print("[case] two_phase_3d_2m, 3 steps with FBM", flush=True)
+    sim = load_simulation(os.path.join(SCEN, "two_phase_3d_2m.config.json"),
+                          os.path.join(SCEN, "two_phase_3d_2m.json"), fbm=True)
+    f = sim.fluid
+    n = f.part_num
+    assert n == 1_953_125
+    rng = np.random.default_rng(0)
+    vf = f.to_numpy("volume_frac").astype(np.float64) + rng.uniform(-0.02, 0.02, (n, 2))
+    vf = np.clip(vf, 0, None)
+    vf = (vf / vf.sum(1, keepdims=True)).astype(np.float32)
+    rho = [np.float32(1000), np.float32(500)]
+    rd = (rho[0] * vf[:, 0] + rho[1] * vf[:, 1]).astype(np.float32)
+    rv = f.to_numpy("rest_volume")
+    for name, a in (("volume_frac", vf), ("rest_density", rd), ("mass", (rd * rv).astype(np.float32))):
+        f.from_numpy(name, a)
+    V = rv.astype(np.float64)
+    pv0 = (V[:, None] * vf.astype(np.float64)).sum(0)
+    sim.sph_step()  # first launches
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        s = sim.sph_step(stats=True)
+        assert 2 <= s.div_iter <= 101 and 2 <= s.incom_iter <= 101
+        assert s.fbm_rounds >= 1
+    torch.cuda.synchronize()
+    print(f"  {(time.perf_counter() - t0) / 3 * 1e3:.2f} ms per step, iterations "
+          f"({s.div_iter}, {s.incom_iter}), FBM rounds {s.fbm_rounds}", flush=True)
+    vf = f.to_numpy("volume_frac")
+    assert np.isfinite(vf).all() and np.isfinite(f.to_numpy("pos")).all() and np.isfinite(f.to_numpy("vel")).all()
+    dev = np.abs(vf.astype(np.float64).sum(1) - 1).max()
+    pv = (V[:, None] * vf.astype(np.float64)).sum(0)
+    drift = np.abs(pv - pv0) / pv0
+    print(f"  max |sum_k phi - 1| {dev:.2e}, min phi {vf.min():.2e}, phase volume drift {drift}", flush=True)
+    assert dev <= 1e-6 and vf.min() >= -1e-7
+    assert drift.max() <= 1e-6
+    rd = (rho[0] * vf[:, 0] + rho[1] * vf[:, 1]).astype(np.float32)
+    assert np.array_equal(f.to_numpy("rest_density"), rd)
+    assert np.array_equal(f.to_numpy("mass"), (rd * rv).astype(np.float32))
