@@ -816,6 +816,97 @@ __global__ void __launch_bounds__(kBlock)
   for (int q = 0; q < P; ++q) tmp[(long)i * P + q] = acc[q];
 }
 
+// The same sweep for NP phases as one cooperative list sweep (kCoop lanes
+// per particle, gathers in flight as in the solver sweeps).  The pair terms
+// of SPH_FBM_convect and SPH_FBM_diffuse share the neighbour's record and
+// W'/r (fast arithmetic); their sums are added to tmp in the reference's
+// order, convection first.  With W'(r) e = (W'/r) x and r W'(r) = r^2 W'/r,
+// the terms are pre_c V_j (f . x) W'/r and pre_d dphi V_j r^2 (W'/r) / (r^2 + eps).
+template <class T, int NP>
+struct FbmRec {
+  V4<T> x;
+  V4<T> u[NP];
+  T phi[NP];
+  T V;
+  int fl;
+};
+
+template <class T, int NP>
+__global__ void __launch_bounds__(kBlock)
+    k_fbm_flux_coop(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
+                    const T* __restrict__ rvol, const T* __restrict__ vf,
+                    const V4<T>* __restrict__ drift, const int* __restrict__ flag,
+                    const int* __restrict__ fstart, const int* __restrict__ nbr_f,
+                    const int* __restrict__ nbr_nf, T* __restrict__ tmp,
+                    const TsphControl* ctrl) {
+  if (!ctrl->general_flag) return;
+  const int base = xcd_block(blockIdx.x, gridDim.x) * kBlock;
+  const int c = coop_lane<kCoop>();
+  const T dt = (T)ctrl->dt;
+#pragma unroll 1
+  for (int round = 0; round < kCoop; ++round) {
+    const int i = coop_particle<kCoop>(base, round);
+    if (i >= n) break;
+    if (flag[i] != 0) continue;  // the whole group: one particle
+    const V4<T> p = fpos[i];
+    V4<T> ui[NP];
+    T pi[NP], conv[NP], diff[NP];
+#pragma unroll
+    for (int s = 0; s < NP; ++s) {
+      ui[s] = drift[(long)i * NP + s];
+      pi[s] = vf[(long)i * NP + s];
+      conv[s] = T(0);
+      diff[s] = T(0);
+    }
+    auto load = [&](int j) {
+      FbmRec<T, NP> d;
+      d.x = fpos[j];
+#pragma unroll
+      for (int s = 0; s < NP; ++s) {
+        d.u[s] = drift[(long)j * NP + s];
+        d.phi[s] = vf[(long)j * NP + s];
+      }
+      d.V = rvol[j];
+      d.fl = flag[j];
+      return d;
+    };
+    auto pair = [&](const FbmRec<T, NP>& d) {
+      if (d.fl != 0) return;
+      const T dx = p.x - d.x.x, dy = p.y - d.x.y, dz = p.z - d.x.z;
+      const T r2 = dx * dx + dy * dy + dz * dz;
+      T r;
+      const T g = f_grad(k, r2, &r);
+      const T gv = g * d.V;
+      const T gd = gv * r2 * f_rcp(r2 + k.eps);
+#pragma unroll
+      for (int s = 0; s < NP; ++s) {
+        const T fx = pi[s] * ui[s].x + d.phi[s] * d.u[s].x;
+        const T fy = pi[s] * ui[s].y + d.phi[s] * d.u[s].y;
+        const T fz = pi[s] * ui[s].z + d.phi[s] * d.u[s].z;
+        conv[s] += (fx * dx + fy * dy + fz * dz) * gv;
+        diff[s] += (pi[s] - d.phi[s]) * gd;
+      }
+    };
+    sweep_lists_coop<kCoop, T, FbmRec<T, NP>>(
+        k, cap, i, c, p, fstart, fstart, fpos, fpos, nbr_f, nullptr, nbr_nf[i], 0, load, load,
+        pair, [&](const FbmRec<T, NP>&) {});
+#pragma unroll
+    for (int s = 0; s < NP; ++s) {
+      conv[s] = group_sum<kCoop>(conv[s]);
+      diff[s] = group_sum<kCoop>(diff[s]);
+    }
+    if (c != 0) continue;
+    const T pre_c = k.fbmC * dt, pre_d = dt * k.fbmD;
+#pragma unroll
+    for (int s = 0; s < NP; ++s) {
+      T a = tmp[(long)i * NP + s];
+      a = a - pre_c * conv[s];
+      a = a + pre_d * diff[s];
+      tmp[(long)i * NP + s] = a;
+    }
+  }
+}
+
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_fbm_check(KP<T> k, int n, const T* __restrict__ vf, const T* __restrict__ tmp,
@@ -1080,10 +1171,18 @@ int fbm_round(const TsphParams& prm, TsphObject* f, TsphControl* ctrl, void* ws,
                      (const V4<T>*)f->vel_adv, (const V4<T>*)f->vel, (const T*)f->rest_density,
                      (const T*)f->volume_frac, (V4<T>*)f->acce_adv, (V4<T>*)f->acce,
                      (T*)f->fbm_zeta, (V4<T>*)f->drift_vel, (T*)f->volume_frac_tmp, ctrl);
-  hipLaunchKernelGGL(k_fbm_flux<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
-                     (const V4<T>*)f->pos, (const T*)f->rest_volume, (const T*)f->volume_frac,
-                     (const V4<T>*)f->drift_vel, f->flag, f->cell_start, f->nbr_f, f->nbr_nf,
-                     (T*)f->volume_frac_tmp, ctrl);
+  auto flux = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
+                       (const V4<T>*)f->pos, (const T*)f->rest_volume, (const T*)f->volume_frac,
+                       (const V4<T>*)f->drift_vel, f->flag, f->cell_start, f->nbr_f, f->nbr_nf,
+                       (T*)f->volume_frac_tmp, ctrl);
+  };
+  switch (prm.phase_num) {  // the cooperative sweep keeps NP phases' records in registers
+    case 1: flux(k_fbm_flux_coop<T, 1>); break;
+    case 2: flux(k_fbm_flux_coop<T, 2>); break;
+    case 3: flux(k_fbm_flux_coop<T, 3>); break;
+    default: flux(k_fbm_flux<T>); break;
+  }
   hipLaunchKernelGGL(k_fbm_check<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n,
                      (const T*)f->volume_frac, (const T*)f->volume_frac_tmp, f->flag, L.partial,
                      ctrl);
