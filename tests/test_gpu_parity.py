@@ -3,7 +3,7 @@
 Integer outputs (cell keys, sort order, grid arrays, neighbour sets) must be
 bit-exact.  Floating-point fields must satisfy
     |gpu - oracle| <= tol * max(|oracle|, RMS(oracle field))
-with tol = 1e-5 for FP32, plus equal solver iteration
+with tol = 1e-5 for FP32 (the single-step bar), plus equal solver iteration
 counts.  The error of both against the FP64 oracle is printed.
 """
 import math
