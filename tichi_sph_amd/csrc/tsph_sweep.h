@@ -68,7 +68,7 @@ __device__ __forceinline__ void walk_cells_batched(const KP<T>& k, const int* __
                                                    F&& f) {
   walk_cells_prefetch<U>(k, cstart, c, [&](int j) { return pos[j]; }, f);
 }
-constexpr int kPassAU = 8;
+constexpr int kPassAU = 4;
 constexpr int kStU = 4;  // prefetch depth of the surface-tension cell walks
 
 template <class T>
