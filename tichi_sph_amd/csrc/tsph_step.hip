@@ -1039,10 +1039,12 @@ int density_alpha(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphContr
   return pass_a<T>(prm, f, b, ctrl, true, L, st);
 }
 
-// The fluid's neighbour lists alone, as pass A builds them: in-range
+// The fluid's neighbour lists alone, as pass A builds them (8 candidates in
+// flight: without pass A's pair math the walk is bound by the gathers): in-range
 // candidates with r > 0 (r^2 > 0) in walk order.  The counters are updated
 // without branches (per-branch increments get merged into an indexed
 // increment of a stack array).
+constexpr int kListsU = 8;
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_lists(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos, const int* __restrict__ fstart,
@@ -1056,13 +1058,13 @@ __global__ void __launch_bounds__(kBlock)
   int c[3];
   node_encode(k, p.x, p.y, p.z, c);
   int nf = 0, nbb = 0;
-  walk_cells_batched<kPassAU>(k, fstart, c, fpos, [&](int j, const V4<T>& q) {
+  walk_cells_batched<kListsU>(k, fstart, c, fpos, [&](int j, const V4<T>& q) {
     const T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
     const bool take = r2 <= k.r2max && r2 > T(0);
     if (take && nf < capf) nbr_f[(long)nf * cap + i] = j;
     nf += take ? 1 : 0;
   });
-  walk_cells_batched<kPassAU>(k, bstart, c, bpos, [&](int j, const V4<T>& q) {
+  walk_cells_batched<kListsU>(k, bstart, c, bpos, [&](int j, const V4<T>& q) {
     const T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
     const bool take = r2 <= k.r2max && r2 > T(0);
     if (take && nbb < capb) nbr_b[(long)nbb * cap + i] = j;
