@@ -168,6 +168,12 @@ def test_fused_step_matches_oracle(name):
     assert s.dt == st.dt
     compare_fields(sim, o, FIELDS_F + ["alpha_1", "alpha_2", "normal"], label="fused")
     compare_fields(sim, o, ["sph_density", "alpha", "W"], obj=1, label="fused")
+    # kappa = psi / alpha of the last Jacobi iteration (the reference forms it
+    # inline in IPPE_update_vel_adv, sph.py:243-244)
+    kap_o = (o.get(0, "psi_adv") / o.get(0, "alpha")).astype(np.float32)
+    e = rel_err(sim.fluid.to_numpy("kappa"), kap_o)
+    print(f"  fused obj0 kappa            rel_err={e:.2e}", flush=True)
+    assert e <= TOL
     for a, b in zip(s.div_err, list(st.div_err)[: st.div_iter]):
         assert a == pytest.approx(b, rel=1e-5)
     # error of GPU and FP32 oracle against the FP64 oracle
