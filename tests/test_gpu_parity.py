@@ -284,6 +284,31 @@ def test_two_phase_fbm_surface_tension_step():
                             "rest_density", "mass"], label="two-phase")
 
 
+def test_two_phase_fbm_reference_surface_step_matches_oracle():
+    """The two-phase VFSPH scene through the sph.py kernels one by one, with
+    the FBM loop (SPH_FBM_clean_tmp / convect / diffuse / check_tmp) and
+    surface tension: the reference surface is bit-exact with the oracle."""
+    sim, o = make("two_phase_3d_4k", fbm=True)
+    rng = np.random.default_rng(1)
+    n = sim.fluid.part_num
+    vf = o.get(0, "volume_frac").astype(np.float64) + rng.uniform(-0.02, 0.02, (n, 2))
+    vf = np.clip(vf, 0, None)
+    vf = (vf / vf.sum(1, keepdims=True)).astype(np.float32)
+    rho = [np.float32(1000), np.float32(500)]
+    rd = (rho[0] * vf[:, 0] + rho[1] * vf[:, 1]).astype(np.float32)
+    m = (rd * o.get(0, "rest_volume")).astype(np.float32)
+    for name, a in (("volume_frac", vf), ("rest_density", rd), ("mass", m)):
+        o.set(0, name, a)
+        sim.fluid.from_numpy(name, a)
+    di, ii, _, _, _ = sim.sph_step_reference()
+    st = o.step(O.STEP_CFL | O.STEP_FBM)
+    assert (di, ii) == (st.div_iter, st.incom_iter)
+    worst = compare_fields(sim, o, ["pos", "vel", "sph_compression", "alpha", "psi_adv", "normal",
+                                    "volume_frac", "rest_density", "mass"], label="two-phase ref-surface")
+    print(f"  largest error {max(worst.values()):.2e}", flush=True)
+    assert max(worst.values()) == 0
+
+
 def test_full_size_sort_and_lists_properties():
     """1M-particle scene: integer invariants that do not need the oracle."""
     from tichi_sph_amd import load_simulation
