@@ -650,6 +650,9 @@ __global__ void __launch_bounds__(kBlock)
 // (sph.py:247-250).  The curvature term carries no kernel weight, so every
 // candidate of the neighbouring cells contributes: this sweep walks the cells.
 // With gamma == 0 every term is an exact zero and only the update runs.
+// Candidate records in flight per lane of the forward surface-tension walk
+// (its records are 44 B; the adjoint walk's 64-B records keep kStU).
+constexpr int kStFwdU = 8;
 template <class T>
 __device__ __forceinline__ void k_surface_tension_body(Blk blk, KP<T> k, int n, const V4<T>* __restrict__ fpos,
                       const T* __restrict__ fmass, const T* __restrict__ psi_rho,
@@ -675,7 +678,7 @@ __device__ __forceinline__ void k_surface_tension_body(Blk blk, KP<T> k, int n, 
     auto load = [&](int j) {
       return Cand{fpos[j], normal[j], fmass[j], psi_rho[j], vf[(long)j * k.P]};
     };
-    walk_cells_prefetch<kStU>(k, fstart, c, load, [&](int, const Cand& d) {
+    walk_cells_prefetch<kStFwdU>(k, fstart, c, load, [&](int, const Cand& d) {
       // every candidate of the 27 cells: reciprocal square root and
       // reciprocal instead of the IEEE square root and divisions (the pair
       // terms of this walk are not part of the bit-exact sums)
@@ -700,6 +703,21 @@ __device__ __forceinline__ void k_surface_tension_body(Blk blk, KP<T> k, int n, 
     const V4<T> v = vel[i];
     vadv[i] = V4<T>{v.x + a.x * dt, v.y + a.y * dt, v.z + a.z * dt, T(0)};
   }
+}
+
+// SPH_advection_update_vel_adv alone (sph.py:247-250), for gamma = 0: a
+// separate kernel, so that it runs at the occupancy of an elementwise kernel
+// rather than at that of the surface-tension walk.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_vel_adv_update(int n, const V4<T>* __restrict__ vel, const V4<T>* __restrict__ acce_adv,
+                     V4<T>* __restrict__ vadv, const TsphControl* ctrl) {
+  TSPH_STRICT
+  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const T dt = (T)ctrl->dt;
+  const V4<T> a = acce_adv[i], v = vel[i];
+  vadv[i] = V4<T>{v.x + a.x * dt, v.y + a.y * dt, v.z + a.z * dt, T(0)};
 }
 
 template <class T>
@@ -1152,10 +1170,14 @@ int advect(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl* ctr
                      (V4<T>*)f->vel_adv, (V4<T>*)f->vel, (V4<T>*)f->acce_adv,
                      (V4<T>*)f->normal);
   const T* psi = (const T*)(prm.solver_type == TSPH_VFSPH ? f->sph_compression : f->sph_density);
-  hipLaunchKernelGGL(k_surface_tension<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n,
-                     (const V4<T>*)f->pos, (const T*)f->mass, psi, (const T*)f->rest_density,
-                     (const T*)f->volume_frac, f->cell_start, (const V4<T>*)f->normal,
-                     (const V4<T>*)f->vel, (V4<T>*)f->acce_adv, (V4<T>*)f->vel_adv, ctrl);
+  if (k.gamma == T(0))
+    hipLaunchKernelGGL(k_vel_adv_update<T>, dim3(nb), dim3(kBlock), 0, st, f->n,
+                       (const V4<T>*)f->vel, (const V4<T>*)f->acce_adv, (V4<T>*)f->vel_adv, ctrl);
+  else
+    hipLaunchKernelGGL(k_surface_tension<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n,
+                       (const V4<T>*)f->pos, (const T*)f->mass, psi, (const T*)f->rest_density,
+                       (const T*)f->volume_frac, f->cell_start, (const V4<T>*)f->normal,
+                       (const V4<T>*)f->vel, (V4<T>*)f->acce_adv, (V4<T>*)f->vel_adv, ctrl);
   TSPH_LAUNCH_CHECK("advect");
   return 0;
 }
