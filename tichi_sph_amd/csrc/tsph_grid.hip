@@ -92,6 +92,18 @@ __global__ void __launch_bounds__(kBlock)
   k_scan_apply_body(Blk{(int)blockIdx.x, (int)gridDim.x}, cnt, len, sums, start);
 }
 
+// The scan's first half with cfl_condition's finish (sph.py:41-47) in its
+// block 0: the CFL partials of k_keys are reduced without a launch of their
+// own (dt is first read by the solver, after the sort).
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_scan_reduce_cfl(const int* __restrict__ cnt, long len, int* __restrict__ sums, KP<T> k,
+                      const double* __restrict__ partial, int nblk, int n, TsphControl* ctrl) {
+  const Blk blk{(int)blockIdx.x, (int)gridDim.x};
+  k_scan_reduce_body(blk, cnt, len, sums);
+  if (blk.b == 0) k_finalize_body<T>(blk, k, (int)kFinCfl, partial, nblk, n, ctrl);
+}
+
 static int launch_scan(const int* cnt, long len, int* sums, int* start, hipStream_t st) {
   long nb = scan_blocks(len);
   if (nb < 1) nb = 1;
@@ -251,17 +263,23 @@ int sort_object(const TsphParams& prm, TsphObject* o, TsphControl* ctrl, int cfl
   hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(len + 1)), dim3(kBlock), 0, st, o->cell_count,
                      len + 1, 0);
   int nb = blocks_for(n);
-  if (n > 0 || cfl) {
-    int nbl = nb > 0 ? nb : 1;
+  const int nbl = nb > 0 ? nb : 1;  // k_keys' blocks (one also for n = 0 with CFL)
+  if (n > 0 || cfl)
     hipLaunchKernelGGL(k_keys<T>, dim3(nbl), dim3(kBlock), 0, st, k, (const V4<T>*)o->pos,
                        (const V4<T>*)o->vel, n, L.key, L.rank, o->cell_count, cfl, L.partial,
                        ctrl);
-    if (cfl)
-      hipLaunchKernelGGL(k_finalize<T>, dim3(1), dim3(kBlock), 0, st, k, (int)kFinCfl, L.partial,
-                         nbl, n, ctrl);
-  }
   TSPH_LAUNCH_CHECK("keys");
-  if (int rc = launch_scan(o->cell_count, len, L.scan_sums, o->cell_start, st)) return rc;
+  if (cfl) {  // the scan, with the CFL partials reduced in its first kernel
+    long nbs = scan_blocks(len);
+    if (nbs < 1) nbs = 1;
+    hipLaunchKernelGGL(k_scan_reduce_cfl<T>, dim3(nbs), dim3(kBlock), 0, st, o->cell_count, len,
+                       L.scan_sums, k, L.partial, nbl, n, ctrl);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nbs), dim3(kBlock), 0, st, o->cell_count, len,
+                       L.scan_sums, o->cell_start);
+    TSPH_LAUNCH_CHECK("scan");
+  } else if (int rc = launch_scan(o->cell_count, len, L.scan_sums, o->cell_start, st)) {
+    return rc;
+  }
   if (n == 0) return 0;
   hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kBlock), 0, st, L.key, L.rank, o->cell_start,
                      o->pid, n, L.tmp_idx, L.tmp_pid, L.tmp_key);
