@@ -143,7 +143,7 @@ class DiffRunner:
         ws, wsb = sim._ws()
         stp = sim.stream_ptr()
         force = self.force[self.step_index] if self.force is not None else (None, None)
-        if sim.bound_dirty:
+        if sim.bound_needs_sort():
             sim._sort_bound()
         sim._call("sort", sim.lib.tsph_sort_object, P, ctypes.byref(f.struct()), sim.ctrl_ptr(), 0, ws, wsb, stp)
         f.adopt_swaps()
@@ -250,16 +250,22 @@ class _Rollout(torch.autograd.Function):
         kept = {}  # segment -> its tapes, for the last segment and those within the budget
         held = []  # (segment, bytes) of the kept earlier segments, oldest first
         used = 0
+        est = None  # bytes of one step's tape, from the first tape recorded
         for t in range(steps):
             s = t // every
             if t % every == 0:
                 ckpts.append(runner.checkpoint())
-                kept[s] = []
-            # the last segment is reversed first: record its tapes now instead
-            # of replaying it in the backward pass; earlier ones while they fit
+                # the last segment is reversed first: record its tapes now
+                # instead of replaying it in the backward pass; an earlier
+                # one only if a whole segment of tapes can fit in the budget
+                seg_len = min(every, steps - t)
+                if s == nseg - 1 or (budget > 0 and (est is None or est * seg_len <= budget)):
+                    kept[s] = []
             tp = runner.step(tape=s in kept)
             if tp is None:
                 continue
+            if est is None:
+                est = tp.nbytes()
             kept[s].append(tp)
             if s == nseg - 1:
                 continue

@@ -46,7 +46,7 @@ class SceneBatch:
                 raise ValueError("every scene of a batch needs fluid particles")
             if s.device != sims[0].device:
                 raise ValueError("all scenes of a batch must live on one device")
-            if s.bound_dirty:
+            if s.bound_needs_sort():
                 s._sort_bound()
         self.sims = list(sims)
         self.lib = sims[0].lib
@@ -54,9 +54,7 @@ class SceneBatch:
         self.cfl = 1 if cfl else 0
         ns = len(sims)
         self.ns = ns
-        self._params = (_lib.TsphParams * ns)(*[s.params for s in sims])
         self._fl = (_lib.TsphObject * ns)(*[s.fluid.struct() for s in sims])
-        self._bo = (_lib.TsphObject * ns)(*[s.bound.struct() for s in sims])
         self._ctrl = (ctypes.c_void_p * ns)(*[s.ctrl.data_ptr() for s in sims])
         self._ws = (ctypes.c_void_p * ns)(*[s.ws.data_ptr() for s in sims])
         self._wsb = (ctypes.c_size_t * ns)(*[s.ws.numel() for s in sims])
@@ -64,11 +62,7 @@ class SceneBatch:
         self.table = torch.zeros(nbytes, dtype=torch.uint8, device=self.device)
         self._staging = torch.zeros(nbytes, dtype=torch.uint8).pin_memory()
         self.info = _lib.TsphBatchInfo()
-        _lib.check(self.lib.tsph_batch_prepare(
-            self._params, self._fl, self._bo, self._ctrl, self._ws, self._wsb, ns,
-            ctypes.c_void_p(self._staging.data_ptr()), ctypes.c_void_p(self.table.data_ptr()), nbytes,
-            ctypes.byref(self.info), self._stream()), "tsph_batch_prepare")
-        torch.cuda.current_stream(self.device).synchronize()  # staging copied
+        self._prepare()
         self._stop_dev = torch.zeros(2 * ns, dtype=torch.int32, device=self.device)
         self._stop_host = torch.zeros(2 * ns, dtype=torch.int32).pin_memory()
         self.kmax = max(s.config.iter_threshold_max for s in sims) + 1
@@ -79,6 +73,30 @@ class SceneBatch:
         self.iter_log = []
 
     # ------------------------------------------------------------ helpers
+    def _prepare(self):
+        """(Re)build the device scene table from the scenes' descriptors."""
+        ns = self.ns
+        self._params = (_lib.TsphParams * ns)(*[s.params for s in self.sims])
+        self._bo = (_lib.TsphObject * ns)(*[s.bound.struct() for s in self.sims])
+        _lib.check(self.lib.tsph_batch_prepare(
+            self._params, self._fl, self._bo, self._ctrl, self._ws, self._wsb, ns,
+            ctypes.c_void_p(self._staging.data_ptr()), ctypes.c_void_p(self.table.data_ptr()),
+            self.table.numel(), ctypes.byref(self.info), self._stream()), "tsph_batch_prepare")
+        torch.cuda.current_stream(self.device).synchronize()  # staging copied
+
+    def _refresh_bounds(self):
+        """A scene whose boundary changed since the table was built (new or
+        moved boundary particles) is re-sorted and the table rebuilt, so no
+        batched step searches a stale boundary cell order."""
+        stale = [s for s in self.sims if s.bound_needs_sort()]
+        if not stale:
+            return
+        if self.graphs is not None:
+            raise RuntimeError("a scene's boundary changed after capture(): release_graphs() first")
+        for s in stale:
+            s._sort_bound()
+        self._prepare()
+
     def _stream(self):
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
@@ -164,6 +182,7 @@ class SceneBatch:
     # ----------------------------------------------------------------- step
     def step(self):
         """One sph_step of every scene; returns ([div_iter], [incom_iter])."""
+        self._refresh_bounds()
         if self.graphs is not None:
             return self._step_graph()
         self._sort()

@@ -17,16 +17,23 @@ import numpy as np
 from .ply import write_ply
 
 
+def _control(sim):
+    """The device control block, where the step (SPH_update_energy,
+    sph.py:272-279) and cfl_condition (sph.py:41-47) leave the energies and
+    dt, or None for a host-only Simulation (nothing has been computed)."""
+    if getattr(sim, "device", None) is not None and sim.device.type == "cuda":
+        return sim.control()
+    return None
+
+
 def _energy(sim):
-    st = getattr(sim, "last_stats", None)
-    ke = st.kinetic_energy if st is not None else 0.0
-    pe = st.gravity_potential_energy if st is not None else 0.0
-    return ke, pe
+    c = _control(sim)
+    return (float(c.ke), float(c.pe)) if c is not None else (0.0, 0.0)
 
 
 def _dt(sim):
-    st = getattr(sim, "last_stats", None)
-    return st.dt if st is not None else sim.config.dt
+    c = _control(sim)
+    return float(c.dt) if c is not None else sim.config.dt
 
 
 def write_full_json(sim, fname):

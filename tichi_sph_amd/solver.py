@@ -386,6 +386,15 @@ class Simulation:
         return steps
 
     # ------------------------------------------------------------ fused step
+    def bound_needs_sort(self):
+        """The boundary's cell order no longer matches its storage: the
+        scene added boundary particles (add_to_scenario, an inlet), or the
+        caller wrote through Fluid.scene_add_* / push_part_seq /
+        from_numpy("pos").  The reference rebuilds its Ngrid every step
+        (main.py:178-183), so a changed boundary is always searched; here the
+        static boundary is re-sorted before the next step instead."""
+        return self.bound_dirty or not self.bound.cell_table_ok
+
     def _sort_bound(self):
         P = ctypes.byref(self.params)
         ws, wsb = self._ws()
@@ -458,7 +467,7 @@ class Simulation:
         P = ctypes.byref(self.params)
         ws, wsb = self._ws()
         st = self.stream_ptr()
-        if self.bound_dirty:
+        if self.bound_needs_sort():
             self._sort_bound()
         self._call("sort", self.lib.tsph_sort_object, P, ctypes.byref(self.fluid.struct()), self.ctrl_ptr(),
                    1 if self.cfl else 0, ws, wsb, st)

@@ -145,8 +145,12 @@ class Fluid:
         self._struct.flags &= ~_lib.TSPH_OBJ_STATIC_SUMS
 
     def from_numpy(self, name, arr):
-        """Write field `name` from an array in reference particle order."""
+        """Write field `name` from an array in reference particle order.
+        Writing positions leaves the storage out of cell order: the next
+        step sorts the object again (for the static boundary too)."""
         self.invalidate_static_sums()
+        if name in ("pos", "neighb_cell_seq", "cell"):
+            self.cell_table_ok = False
         pid = self.t["pid"][: self.part_num].cpu().numpy()
         a = np.asarray(arr)[pid]
         dst = self._storage(name)
@@ -248,7 +252,8 @@ class Fluid:
         self.t["pid"][sl] = torch.arange(n0, n1, dtype=torch.int32, device=self.device)
         self.color += [color] * int(pushed_part_num)
         self.part_num = n1
-        self.cell_table_ok = False
+        self.cell_table_ok = False  # the next step sorts the object again
+        self.invalidate_static_sums()
         self.update_mass_all()
 
     def update_mass_all(self):
