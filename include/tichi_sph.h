@@ -163,6 +163,28 @@ int tsph_sort_object(const TsphParams* p, TsphObject* obj, TsphControl* ctrl, in
                      void* workspace, size_t ws_bytes, void* stream);
 
 /*
+ * tsph_sort_object without a workspace.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory or pass a workspace
+ * (code under hipGraph capture, tight serving loops, harnesses that forbid
+ * allocation).  The sort's scratch (keys, arrival ranks, provisional slots,
+ * scan sums, CFL partials) lives in a file-scope device array of
+ * tsph_sort_static_scratch_bytes() bytes (about 42 MB) that each device
+ * holds from the library's first use on it until the process ends; its
+ * address is looked up once per device, on the first call there.  The call
+ * itself allocates nothing.  Limits: obj->n <= tsph_sort_static_max_particles()
+ * (2^21) and node_num <= tsph_sort_static_max_cells() (2^24); larger inputs
+ * return TSPH_ECAPACITY without launching anything.  Every call on a device
+ * shares that device's array, so calls that overlap on one device (other
+ * streams, other host threads) race; calls ordered on one stream do not.
+ * Results are bit-identical to tsph_sort_object's.
+ */
+int tsph_sort_object_static(const TsphParams* p, TsphObject* obj, TsphControl* ctrl, int32_t cfl,
+                            void* stream);
+size_t tsph_sort_static_scratch_bytes(void);
+int32_t tsph_sort_static_max_particles(void);
+int32_t tsph_sort_static_max_cells(void);
+
+/*
  * The reference's shared Ngrid arrays (sph_obj.py:283-293), built over the
  * objects' current storage order: node_part_count/shift/shift_count per cell
  * and part_pid_in_node/part_uid_in_node per slot, fluid entries before

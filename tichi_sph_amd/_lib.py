@@ -111,7 +111,8 @@ EXPORTED = [
     "tsph_batch_table_bytes", "tsph_batch_prepare", "tsph_batch_sort", "tsph_batch_density_alpha",
     "tsph_batch_solver_begin", "tsph_batch_solver_psi", "tsph_batch_solver_update",
     "tsph_batch_advect", "tsph_batch_tail", "tsph_batch_stop_state", "tsph_map_velocity",
-    "tsph_wcsph",
+    "tsph_wcsph", "tsph_sort_object_static", "tsph_sort_static_scratch_bytes",
+    "tsph_sort_static_max_particles", "tsph_sort_static_max_cells",
 ]
 
 _lib = None
@@ -145,9 +146,13 @@ def load(path=None):
     BI = ctypes.POINTER(TsphBatchInfo)
     lib.tsph_batch_table_bytes.restype = ctypes.c_size_t
     lib.tsph_batch_table_bytes.argtypes = [_i32, _i32]
+    lib.tsph_sort_static_scratch_bytes.restype = ctypes.c_size_t
+    lib.tsph_sort_static_max_particles.restype = _i32
+    lib.tsph_sort_static_max_cells.restype = _i32
     sz = ctypes.c_size_t
     sigs = {
         "tsph_sort_object": [P, O, C, _i32, _vp, sz, _vp],
+        "tsph_sort_object_static": [P, O, C, _i32, _vp],
         "tsph_ngrid_build": [P, ctypes.POINTER(O), _i32, _vp, _vp, _vp, _vp, _vp, _vp, sz, _vp],
         "tsph_neighbor_lists": [P, O, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
         "tsph_density_alpha": [P, O, O, C, _vp, sz, _vp],

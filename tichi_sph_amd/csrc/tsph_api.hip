@@ -21,6 +21,11 @@ void set_error(const char* fmt, ...) {
 template <class T>
 int sort_object(const TsphParams&, TsphObject*, TsphControl*, int, void*, size_t, hipStream_t);
 template <class T>
+int sort_object_static(const TsphParams&, TsphObject*, TsphControl*, int, hipStream_t);
+size_t static_sort_scratch_bytes();
+long static_sort_max_n();
+long static_sort_max_cells();
+template <class T>
 int ngrid_build(const TsphParams&, TsphObject* const*, int, int*, int*, int*, int*, int*, void*,
                 size_t, hipStream_t);
 template <class T>
@@ -248,6 +253,17 @@ int tsph_sort_object(const TsphParams* p, TsphObject* obj, TsphControl* ctrl, in
   if (int rc = check_obj(obj, "tsph_sort_object")) return rc;
   return DISPATCH(p, sort_object, *p, obj, ctrl, cfl, ws, wsb, (hipStream_t)stream);
 }
+
+int tsph_sort_object_static(const TsphParams* p, TsphObject* obj, TsphControl* ctrl, int32_t cfl,
+                            void* stream) {
+  if (int rc = check_params(p)) return rc;
+  if (int rc = check_obj(obj, "tsph_sort_object_static")) return rc;
+  return DISPATCH(p, sort_object_static, *p, obj, ctrl, cfl, (hipStream_t)stream);
+}
+
+size_t tsph_sort_static_scratch_bytes(void) { return tsph::static_sort_scratch_bytes(); }
+int32_t tsph_sort_static_max_particles(void) { return (int32_t)tsph::static_sort_max_n(); }
+int32_t tsph_sort_static_max_cells(void) { return (int32_t)tsph::static_sort_max_cells(); }
 
 int tsph_ngrid_build(const TsphParams* p, TsphObject* const* objs, int32_t nobj, int32_t* npc,
                      int32_t* nps, int32_t* npsc, int32_t* pin, int32_t* uin, void* ws, size_t wsb,

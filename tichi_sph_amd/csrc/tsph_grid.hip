@@ -14,6 +14,8 @@
 //             into the back buffers.
 // Reference layout (tsph_ngrid_build): the reference's shared Ngrid arrays
 // over unsorted storage, same algorithm over (cell, uid) buckets.
+#include <mutex>
+
 #include "tsph_host.h"
 
 namespace tsph {
@@ -250,14 +252,21 @@ static void swap_state(TsphObject* o) {
   std::swap(o->pid, o->pid_b);
 }
 
+// The scratch of one counting sort: CFL block partials, scan tile sums, and
+// per-particle keys, arrival ranks and the scatter's provisional slots.
+// Every array is written before it is read within one sort, so nothing
+// needs clearing between calls.
+struct SortScratch {
+  double* partial;  // [2 * (blocks_for(n) + 8)]
+  int* scan_sums;   // [scan_blocks(node_num + 2) + 8]
+  int *key, *rank, *tmp_idx, *tmp_pid, *tmp_key;  // [n]
+};
+
 template <class T>
-int sort_object(const TsphParams& prm, TsphObject* o, TsphControl* ctrl, int cfl, void* ws,
-                size_t wsb, hipStream_t st) {
+static int sort_object_core(const TsphParams& prm, TsphObject* o, TsphControl* ctrl, int cfl,
+                            const SortScratch& L, hipStream_t st) {
   KP<T> k = make_kp<T>(prm);
   int n = o->n;
-  WsLayout L;
-  TSPH_REQUIRE(carve_ws(ws, wsb, o->capacity, prm.node_num, &L), TSPH_ECAPACITY,
-               "tsph_sort_object: workspace too small (%zu bytes)", wsb);
   o->flags &= ~TSPH_OBJ_STATIC_SUMS;  // new storage order: the kept sums no longer apply
   long len = (long)prm.node_num + 1;  // cells + trailing invalid bucket
   hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(len + 1)), dim3(kBlock), 0, st, o->cell_count,
@@ -292,6 +301,86 @@ int sort_object(const TsphParams& prm, TsphObject* o, TsphControl* ctrl, int cfl
   TSPH_LAUNCH_CHECK("permute");
   swap_state(o);
   return 0;
+}
+
+template <class T>
+int sort_object(const TsphParams& prm, TsphObject* o, TsphControl* ctrl, int cfl, void* ws,
+                size_t wsb, hipStream_t st) {
+  WsLayout W;
+  TSPH_REQUIRE(carve_ws(ws, wsb, o->capacity, prm.node_num, &W), TSPH_ECAPACITY,
+               "tsph_sort_object: workspace too small (%zu bytes)", wsb);
+  const SortScratch L{W.partial, W.scan_sums, W.key, W.rank, W.tmp_idx, W.tmp_pid, W.tmp_key};
+  return sort_object_core<T>(prm, o, ctrl, cfl, L, st);
+}
+
+// ---------------------------------------------------------------------------
+// tsph_sort_object_static: the same sort with its scratch in a file-scope
+// device array instead of the caller's workspace, for callers that cannot
+// allocate.  Each device holds its own copy of `scratch` (the runtime places
+// it when this code object is loaded on the device, for the life of the
+// process); its device address is looked up once per device.  Calls that
+// overlap on one device share the array and therefore race.
+// ---------------------------------------------------------------------------
+constexpr long kStaticSortMaxN = 1L << 21;      // particles of the object
+constexpr long kStaticSortMaxCells = 1L << 24;  // node_num
+constexpr size_t cseg(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kStaticPartialBytes = cseg(sizeof(double) * 2 * (kStaticSortMaxN / kBlock + 8));
+constexpr size_t kStaticScanBytes =
+    cseg(sizeof(int) * ((kStaticSortMaxCells + 2 + kScanTile - 1) / kScanTile + 8));
+constexpr size_t kStaticIdxBytes = cseg(sizeof(int) * kStaticSortMaxN);
+constexpr size_t kStaticScratchBytes = kStaticPartialBytes + kStaticScanBytes + 5 * kStaticIdxBytes;
+
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+constexpr int kStaticMaxDevices = 64;
+static void* g_scratch_dev[kStaticMaxDevices];
+static std::mutex g_scratch_mu;
+
+// The device address of `scratch` on the current device (host code must not
+// use the array's name as a device address: it is a host-side stand-in).
+static int static_scratch(void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  TSPH_REQUIRE(e == hipSuccess, TSPH_EHIP, "tsph_sort_object_static: hipGetDevice: %s",
+               hipGetErrorString(e));
+  TSPH_REQUIRE(dev >= 0 && dev < kStaticMaxDevices, TSPH_EUNSUPPORTED,
+               "tsph_sort_object_static: device %d beyond the %d supported", dev, kStaticMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    TSPH_REQUIRE(e == hipSuccess && ptr != nullptr, TSPH_EHIP,
+                 "tsph_sort_object_static: hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_dev[dev] = ptr;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+size_t static_sort_scratch_bytes() { return kStaticScratchBytes; }
+long static_sort_max_n() { return kStaticSortMaxN; }
+long static_sort_max_cells() { return kStaticSortMaxCells; }
+
+template <class T>
+int sort_object_static(const TsphParams& prm, TsphObject* o, TsphControl* ctrl, int cfl,
+                       hipStream_t st) {
+  TSPH_REQUIRE(o->n <= kStaticSortMaxN, TSPH_ECAPACITY,
+               "tsph_sort_object_static: %d particles exceed the static scratch's maximum of %ld",
+               o->n, kStaticSortMaxN);
+  TSPH_REQUIRE((long)prm.node_num <= kStaticSortMaxCells, TSPH_ECAPACITY,
+               "tsph_sort_object_static: node_num %d exceeds the static scratch's maximum of %ld",
+               prm.node_num, kStaticSortMaxCells);
+  void* base = nullptr;
+  if (int rc = static_scratch(&base)) return rc;
+  char* p = (char*)base;
+  SortScratch L;
+  L.partial = (double*)p;
+  p += kStaticPartialBytes;
+  L.scan_sums = (int*)p;
+  p += kStaticScanBytes;
+  int** idx[5] = {&L.key, &L.rank, &L.tmp_idx, &L.tmp_pid, &L.tmp_key};
+  for (int q = 0; q < 5; ++q, p += kStaticIdxBytes) *idx[q] = (int*)p;
+  return sort_object_core<T>(prm, o, ctrl, cfl, L, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -533,6 +622,8 @@ template int batch_sort<double>(const TsphBatchInfo&, char*, TsphObject*, int, i
 
 template int sort_object<float>(const TsphParams&, TsphObject*, TsphControl*, int, void*, size_t, hipStream_t);
 template int sort_object<double>(const TsphParams&, TsphObject*, TsphControl*, int, void*, size_t, hipStream_t);
+template int sort_object_static<float>(const TsphParams&, TsphObject*, TsphControl*, int, hipStream_t);
+template int sort_object_static<double>(const TsphParams&, TsphObject*, TsphControl*, int, hipStream_t);
 template int ngrid_build<float>(const TsphParams&, TsphObject* const*, int, int*, int*, int*, int*, int*, void*, size_t, hipStream_t);
 template int ngrid_build<double>(const TsphParams&, TsphObject* const*, int, int*, int*, int*, int*, int*, void*, size_t, hipStream_t);
 
