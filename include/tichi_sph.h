@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define TSPH_ABI_VERSION 3
+#define TSPH_ABI_VERSION 4
 #define TSPH_MAX_PHASE 8
 #define TSPH_MAX_ITER_LOG 256
 
@@ -68,6 +68,12 @@ typedef struct TsphParams {
   double r2_max; /* largest r^2 whose fl(fl(sqrt(r^2))/h) < 1 in this precision */
   double dynamic_viscosity, surface_tension_gamma, fbm_diffusion, fbm_convection, cfl_factor;
   double divergence_threshold, compression_threshold;
+  /* Per-phase dynamic viscosity, an extension (the reference has one global
+   * mu, sph_config.py:148): with mu_per_phase != 0 the viscous force of
+   * particle i uses mu_i = sum_k volume_frac_ik * phase_viscosity[k]
+   * (k ascending, each term rounded once) in place of dynamic_viscosity. */
+  double phase_viscosity[TSPH_MAX_PHASE];
+  int32_t mu_per_phase, pad2;
 } TsphParams;
 
 /*
@@ -359,6 +365,16 @@ typedef struct TsphAdjoint {
 int tsph_solver_psi_taped(const TsphParams* p, TsphObject* fluid, TsphObject* bound,
                           TsphControl* ctrl, int32_t k, void* tape_vel_adv, void* tape_kappa,
                           void* workspace, size_t ws_bytes, void* stream);
+
+/* tsph_solver_psi with the clamp set of a recorded iteration: psi_adv is
+ * set to 0 exactly where kappa_ref (a tape_kappa of tsph_solver_psi_taped,
+ * in the same cell order) is 0, and left unclamped elsewhere, instead of
+ * clamping the negative values (IPPE_psi_adv_non_negative, sph.py:219-226).
+ * It replays the path whose derivative the adjoint takes, so that finite
+ * differences of replayed steps do not cross the clamp's kink. */
+int tsph_solver_psi_frozen(const TsphParams* p, TsphObject* fluid, TsphObject* bound,
+                           TsphControl* ctrl, int32_t k, const void* kappa_ref, void* workspace,
+                           size_t ws_bytes, void* stream);
 
 /* The fluid's neighbour lists and counts of tsph_density_alpha alone,
  * rebuilt (bit-identically) from recorded sorted positions and cell table,

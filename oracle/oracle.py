@@ -33,6 +33,7 @@ class RawInit(ctypes.Structure):
         ("cfl_factor", ctypes.c_double),
         ("fbm_diffusion", ctypes.c_double), ("fbm_convection", ctypes.c_double),
         ("surface_tension_gamma", ctypes.c_double),
+        ("mu_per_phase", ctypes.c_int), ("phase_viscosity", ctypes.c_double * MAXP),
     ]
 
 
@@ -117,6 +118,11 @@ def raw_init(config_buffer, scenario_buffer):
     r.fbm_diffusion = cb["solver_fbm_diffusion_term"]
     r.fbm_convection = cb["solver_fbm_convection_term"]
     r.surface_tension_gamma = cb["solver_surface_tension_gamma"]
+    mus = env.get("phase_dynamic_viscosity")  # the per-phase viscosity extension
+    if mus is not None:
+        r.mu_per_phase = 1
+        for k in range(r.phase_num):
+            r.phase_viscosity[k] = mus[k]
     return r
 
 

@@ -72,6 +72,9 @@ struct RawInit {
   int cs;
   double cfl_factor;
   double fbm_diffusion, fbm_convection, surface_tension_gamma;
+  // extension: per-phase viscosities (mu_per_phase != 0), see Consts
+  int mu_per_phase;
+  double phase_viscosity[kMaxPhase];
 };
 
 inline double f32(double x) { return (double)(float)x; }
@@ -83,6 +86,11 @@ struct Consts {
   double part_size[5] = {0}, kernel_h[5] = {0}, kernel_sig[4] = {0};
   double lb[3] = {0}, rt[3] = {0}, g[3] = {0}, rho[kMaxPhase] = {0};
   double mu = 0;
+  // Per-phase viscosity, an extension the reference lacks (one global mu,
+  // sph_config.py:148): mu_i = sum_k phi_ik mu_k replaces mu in
+  // SPH_advection_viscosity_acc when mu_pp is set.
+  int mu_pp = 0;
+  double muk[kMaxPhase] = {0};
   double grid_size[3] = {0};
   int coder[3] = {0, 0, 0};
   int node_num = 0;
@@ -102,6 +110,8 @@ Consts make_consts(const RawInit& in) {
   // init_sim_env, sph_config.py:145-163
   c.part_size[1] = f32(in.global_part_size);
   c.mu = f32(in.dynamic_viscosity);
+  c.mu_pp = in.mu_per_phase;
+  for (int k = 0; k < c.P; ++k) c.muk[k] = f32(in.phase_viscosity[k]);
   for (int d = 0; d < 3; ++d) {
     c.lb[d] = d < c.dim ? f32(in.lb[d]) : 0.0;
     c.rt[d] = d < c.dim ? f32(in.rt[d]) : 0.0;
@@ -608,6 +618,12 @@ struct Sim {
     T two_d2 = T(2 * (2 + d));
     T eps = T(0.01) * km.h2;
     par_for(o.n, [&](int i) {
+      T mu = T(c.mu);
+      if (c.mu_pp) {  // extension: mu_i = sum_k phi_ik mu_k, k ascending
+        const T* phi = &o.volume_frac[(size_t)i * c.P];
+        mu = phi[0] * T(c.muk[0]);
+        for (int k = 1; k < c.P; ++k) mu = mu + phi[k] * T(c.muk[k]);
+      }
       walk(o, i, no, [&](int j) {
         T x[3], A[3];
         for (int k = 0; k < d; ++k) {
@@ -626,7 +642,7 @@ struct Sim {
           for (int k = 0; k < d; ++k) {
             T lap = s * (inv * x[k]) * Ax / den;
             o.acce_adv[(size_t)i * d + k] =
-                o.acce_adv[(size_t)i * d + k] + lap * T(c.mu) / o.rest_density[i];
+                o.acce_adv[(size_t)i * d + k] + lap * mu / o.rest_density[i];
           }
         }
       });

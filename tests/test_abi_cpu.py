@@ -26,14 +26,15 @@ def test_library_exports_every_declared_symbol():
     assert set(decl) == set(_lib.EXPORTED)
     for name in decl:
         assert hasattr(lib, name), name
-    assert lib.tsph_abi_version() == _lib.ABI_VERSION == 3
+    assert lib.tsph_abi_version() == _lib.ABI_VERSION == 4
 
 
 def test_struct_layout_matches_header(tmp_path):
     from tichi_sph_amd import _lib
     fields = {
         "TsphParams": ["dim", "list_cap_f", "grid_size", "cs", "list_cap_b", "lb", "phase_rest_density",
-                       "part_size", "kernel_sig", "r2_max", "cfl_factor", "compression_threshold"],
+                       "part_size", "kernel_sig", "r2_max", "cfl_factor", "compression_threshold",
+                       "phase_viscosity", "mu_per_phase"],
         "TsphObject": ["n", "flags", "pos", "pressure", "pid", "cell_structured", "cell_start", "nbr_nb", "pid_b",
                        "static_sums"],
         "TsphControl": ["dt", "it", "stop", "general_flag", "ke", "run_u", "err_div", "err_incom", "signal", "token",

@@ -111,13 +111,15 @@ def test_adjoint_matches_central_fd_fp64(name, steps, every):
 def test_adjoint_at_differentiable_dam_break_size_fp64():
     """The 200k-particle differentiable dam break (target from mu = 0.02,
     gamma = 0.05; start mu = 0.01, gamma = 0.07, v0 ~ N(0, 0.01^2)) over 20
-    steps checkpointed every 5, against FP64 central differences.
-
-    The v0 direction is the hardest case: measured distance of the FD
-    quotient to the adjoint for velocity steps 1e-8 / 1e-9 / 1e-10 / 1e-11 /
-    1e-12 is 4.2e-2 / 4.2e-3 / 8.5e-4 / 1.0e-2 / 1.0e-2 (psi-clamp kinks
-    above 1e-10; below it round-off, the loss being ~1e-12), so it uses
-    1e-10 and a bar of 2e-3; mu and gamma agree to ~5e-5."""
+    steps checkpointed every 5, against FP64 central differences along the
+    recorded path: the replayed steps clamp psi where the recorded iterations
+    did (tsph_solver_psi_frozen), which is the function whose derivative the
+    adjoint takes.  Differences of the free-running step are printed beside
+    them: some of the 200k particles sit at the clamp's kink at every scale,
+    so along v0 they reach only ~1e-3 (4.2e-2 / 4.2e-3 / 8.5e-4 / 1.0e-2 for
+    steps 1e-8 / 1e-9 / 1e-10 / 1e-11).  Steps 3e-8 (v0) and 3e-3 relative
+    (mu, gamma): along the recorded path the differences converge as the
+    step grows, round-off dominating below about 1e-9.  Bar 1e-3."""
     steps, every = 20, 5
     r, st0, v0, target, rollout = setup_case("diff_3d_200k", steps)
     mu = torch.tensor(0.01, dtype=torch.float64, device=v0.device, requires_grad=True)
@@ -127,21 +129,38 @@ def test_adjoint_at_differentiable_dam_break_size_fp64():
     loss_of(x, target).backward()
     base_log = list(r.iter_log)
     print(f"  iterations {base_log}", flush=True)
+    # the recorded path's clamp sets
+    r.restore(st0)
+    r.set_params(0.01, 0.07)
+    r.set_velocity_ref(v0)
+    r.force, r.step_index = base_log, 0
+    tapes = [r.step(tape=True) for _ in range(steps)]
+    r.force = None
 
-    def L_at(mu_, gam_, v_):
-        return float(loss_of(forward_only(r, st0, v_, mu_, gam_, steps, force=base_log), target))
+    def L_at(mu_, gam_, v_, frozen=True):
+        r.restore(st0)
+        r.set_params(mu_, gam_)
+        r.set_velocity_ref(v_)
+        r.force, r.step_index = base_log, 0
+        for t in range(steps):
+            r.step(frozen=tapes[t] if frozen else None)
+        r.force = None
+        return float(loss_of(r.positions_ref(), target))
 
     g = torch.Generator().manual_seed(3)
     dv = torch.randn(v0.shape, generator=g, dtype=torch.float64).to(v0.device)
-    checks = [("mu", float(mu.grad), (L_at(0.01 + 1e-6, 0.07, v0) - L_at(0.01 - 1e-6, 0.07, v0)) / 2e-6),
-              ("gamma", float(gam.grad), (L_at(0.01, 0.07 + 7e-6, v0) - L_at(0.01, 0.07 - 7e-6, v0)) / 1.4e-5),
+    em, eg, ev = 3e-3 * 0.01, 3e-3 * 0.07, 3e-8
+    checks = [("mu", float(mu.grad), (L_at(0.01 + em, 0.07, v0) - L_at(0.01 - em, 0.07, v0)) / (2 * em)),
+              ("gamma", float(gam.grad), (L_at(0.01, 0.07 + eg, v0) - L_at(0.01, 0.07 - eg, v0)) / (2 * eg)),
               ("v0 . dv", float((v.grad * dv).sum()),
-               (L_at(0.01, 0.07, v0 + 1e-10 * dv) - L_at(0.01, 0.07, v0 - 1e-10 * dv)) / 2e-10)]
+               (L_at(0.01, 0.07, v0 + ev * dv) - L_at(0.01, 0.07, v0 - ev * dv)) / (2 * ev))]
+    free = (L_at(0.01, 0.07, v0 + 1e-10 * dv, False) - L_at(0.01, 0.07, v0 - 1e-10 * dv, False)) / 2e-10
     bad = []
     for what, ad, fd in checks:
         print(f"  {what:8s} adjoint={ad:+.10e} fd={fd:+.10e} rel={rel(ad, fd):.2e}", flush=True)
-        if rel(ad, fd) > (2e-3 if what == "v0 . dv" else FD_TOL):
+        if rel(ad, fd) > FD_TOL:
             bad.append(what)
+    print(f"  v0 . dv free-running fd (step 1e-10) {free:+.10e} rel {rel(checks[2][1], free):.2e}", flush=True)
     assert not bad, bad
 
 

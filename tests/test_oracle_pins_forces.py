@@ -438,3 +438,68 @@ def test_cfl_velocity_branch_closed_form(precision):
     got = o.scalar("dt")
     assert expect < dt
     assert got == float(expect)
+
+
+# ------------------------------------------------- per-phase viscosity (extension)
+
+
+def test_per_phase_viscosity_two_particle_closed_form():
+    """The per-phase viscosity extension (scenario key phase_dynamic_viscosity;
+    the reference has one global mu, sph_config.py:148): the viscous term of
+    sph.py:128-129 with mu replaced by particle i's mu_i = sum_k phi_ik mu_k.
+    Two particles with different fractions, so that using mu_j, or the
+    global mu, changes the result."""
+    base = np.array([0.05, 0.05, 0.05])
+    dx = np.array([0.006, -0.003, 0.002])
+    o, cb = small_oracle("two_phase_3d_4k", [base, base + dx])
+    run(o, *GRID, ("SPH_clean_value", (0,)))
+    phi = np.array([[0.8, 0.2], [0.3, 0.7]])
+    V = np.array([5e-7, 6e-7])
+    psi = np.array([0.98, 1.04])
+    rho0 = phi @ np.array([1000.0, 500.0])
+    v = np.array([[0.4, -0.1, 0.2], [-0.3, 0.5, 0.0]])
+    for name, a in (("volume_frac", phi), ("rest_volume", V), ("sph_compression", psi),
+                    ("rest_density", rho0), ("vel", v)):
+        o.set(0, name, a)
+    o.kernel("SPH_advection_viscosity_acc", 0, 0)
+    _, sb = load_scene("two_phase_3d_4k")
+    muk = np.array([float(np.float32(m)) for m in sb["sim_env"]["phase_dynamic_viscosity"]])
+    assert muk[0] != muk[1]
+    h, sig = o.c.kernel_h[1], o.c.kernel_sig[3]
+    x = o.get(0, "pos")
+    got = o.get(0, "acce_adv")
+    for i, j in ((0, 1), (1, 0)):
+        xij = x[i] - x[j]
+        r = np.linalg.norm(xij)
+        wg = spline(r, h, sig)[0]
+        Vj = V[j] / psi[j]
+        lap = 2 * (2 + 3) * Vj * wg * (xij / r) * ((v[i] - v[j]) @ xij) / (0.01 * o.c.kernel_h[2] + r * r)
+        mu_i = phi[i] @ muk
+        assert got[i] == pytest.approx(lap * mu_i / rho0[i], rel=1e-10), i
+
+
+@pytest.mark.parametrize("precision", [4, 8])
+def test_per_phase_viscosity_reduces_to_global_for_pure_phases(precision):
+    """Particles holding one phase only: mu_i = 1 * mu_k + 0 * mu_other = mu_k
+    exactly, so the extension's viscous forces equal, bit for bit, those of a
+    global mu = mu_k (the reference's kernel)."""
+    cb, sb = load_scene("two_phase_3d_4k")
+    import json as _json
+    mus = sb["sim_env"]["phase_dynamic_viscosity"]
+    sb_glob = _json.loads(_json.dumps(sb))
+    del sb_glob["sim_env"]["phase_dynamic_viscosity"]
+    sb_glob["sim_env"]["global_dynamic_viscosity"] = mus[1]
+    out = []
+    for s in (sb, sb_glob):
+        o = O.Oracle(cb, s, precision)
+        o.init_scenario()
+        n = o.n[0]
+        pure = np.zeros((n, 2))
+        pure[:, 1] = 1.0  # every particle phase B
+        o.set(0, "volume_frac", pure)
+        o.set(0, "vel", np.random.default_rng(4).normal(0, 0.1, (n, 3)))
+        run(o, *GRID, ("SPH_clean_value", (0,)), ("SPH_prepare_attr", (0, 0)), ("SPH_prepare_attr", (0, 1)),
+            ("SPH_advection_viscosity_acc", (0, 0)))
+        out.append(o.get(0, "acce_adv"))
+    assert np.abs(out[0]).max() > 0
+    assert np.array_equal(out[0], out[1])

@@ -9,7 +9,7 @@ import os
 
 TSPH_MAX_PHASE = 8
 TSPH_MAX_ITER_LOG = 256
-ABI_VERSION = 3  # TSPH_ABI_VERSION of include/tichi_sph.h
+ABI_VERSION = 4  # TSPH_ABI_VERSION of include/tichi_sph.h
 TSPH_FLAG_FAST_QUOTIENT = 1  # TsphParams.flags
 TSPH_OBJ_STATIC_SUMS = 1  # TsphObject.flags
 TSPH_DFSPH = 0
@@ -38,6 +38,7 @@ class TsphParams(ctypes.Structure):
         ("dynamic_viscosity", _f64), ("surface_tension_gamma", _f64), ("fbm_diffusion", _f64),
         ("fbm_convection", _f64), ("cfl_factor", _f64),
         ("divergence_threshold", _f64), ("compression_threshold", _f64),
+        ("phase_viscosity", _f64 * TSPH_MAX_PHASE), ("mu_per_phase", _i32), ("pad2", _i32),
     ]
 
 
@@ -106,7 +107,7 @@ EXPORTED = [
     "tsph_verify_fast_quotient",
     "tsph_solver_iteration", "tsph_solver_psi", "tsph_solver_update", "tsph_advect",
     "tsph_fbm_round", "tsph_tail",
-    "tsph_ref_kernel", "tsph_solver_psi_taped", "tsph_relist", "tsph_adj_begin",
+    "tsph_ref_kernel", "tsph_solver_psi_taped", "tsph_solver_psi_frozen", "tsph_relist", "tsph_adj_begin",
     "tsph_adj_solver_iteration", "tsph_adj_advect", "tsph_adj_density_alpha", "tsph_adj_end",
     "tsph_batch_table_bytes", "tsph_batch_prepare", "tsph_batch_sort", "tsph_batch_density_alpha",
     "tsph_batch_solver_begin", "tsph_batch_solver_psi", "tsph_batch_solver_update",
@@ -169,6 +170,7 @@ def load(path=None):
         "tsph_tail": [P, O, C, _i32, _vp, sz, _vp],
         "tsph_ref_kernel": [P, _i32, O, O, _vp, _vp, _vp, _vp, C, _vp, sz, _vp],
         "tsph_solver_psi_taped": [P, O, O, C, _i32, _vp, _vp, _vp, sz, _vp],
+        "tsph_solver_psi_frozen": [P, O, O, C, _i32, _vp, _vp, sz, _vp],
         "tsph_relist": [P, O, O, C, _vp, sz, _vp],
         "tsph_adj_begin": [P, O, C, A, _vp, _vp, _vp, _vp],
         "tsph_adj_solver_iteration": [P, O, O, C, A, _vp, sz, _vp],

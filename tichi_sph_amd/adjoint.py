@@ -67,6 +67,8 @@ class DiffRunner:
             raise ValueError("the adjoint does not differentiate FBM rounds")
         if sim.cfl:
             raise ValueError("the adjoint needs a fixed dt: build the Simulation with cfl=False")
+        if cfg.phase_dynamic_viscosity is not None:
+            raise ValueError("the adjoint differentiates the global viscosity only (no per-phase viscosity)")
         self.sim = sim
         self.lib = sim.lib
         f = sim.fluid
@@ -134,9 +136,13 @@ class DiffRunner:
         return out
 
     # ------------------------------------------------------------- forward
-    def step(self, tape=False):
+    def step(self, tape=False, frozen=None):
         """One fused step (Simulation.sph_step without CFL / FBM); with
-        tape=True returns the StepTape of the step."""
+        tape=True returns the StepTape of the step.  With `frozen` (a StepTape
+        of this step, recorded earlier) the Jacobi solves clamp psi where the
+        recorded iterations clamped it (tsph_solver_psi_frozen): the step then
+        follows the recorded path, whose derivative the adjoint takes, also
+        under a perturbed input (finite differences along that path)."""
         sim = self.sim
         f, b = sim.fluid, sim.bound
         P = ctypes.byref(sim.params)
@@ -163,12 +169,15 @@ class DiffRunner:
             tp.alpha = f.t["alpha"][:n].clone()
             tp.vol = f.t["vol"][:n].clone()
             tp.alpha_1 = f.t["alpha_1"][:n].clone()
-        d_it = sim._solve(0, tape=tp.iters[0] if tape else None, force=force[0])
+        fz = [None, None] if frozen is None else [[kap for _, kap in frozen.iters[q]] for q in (0, 1)]
+        if frozen is not None and force[0] is None:
+            force = (len(fz[0]), len(fz[1]))
+        d_it = sim._solve(0, tape=tp.iters[0] if tape else None, force=force[0], frozen=fz[0])
         sim._call("advect", sim.lib.tsph_advect, P, fs, bs, sim.ctrl_ptr(), ws, wsb, stp)
         if tape:
             tp.vel = f.t["vel"][:n].clone()
             tp.normal = f.t["normal"][:n].clone()
-        i_it = sim._solve(1, tape=tp.iters[1] if tape else None, force=force[1])
+        i_it = sim._solve(1, tape=tp.iters[1] if tape else None, force=force[1], frozen=fz[1])
         sim._call("tail", sim.lib.tsph_tail, P, fs, sim.ctrl_ptr(), 0, ws, wsb, stp)
         if tape:
             tp.calls = sim.calls  # no later call: lists and density / alpha still resident

@@ -96,6 +96,13 @@ class Config:
         if len(rho) != self.phase_num:
             raise Exception("phase_rest_density must have phase_num entries")
         self.phase_rest_density = [_f32(v) for v in rho]
+        # Extension (not in the reference, which has one global mu,
+        # sph_config.py:148): per-phase dynamic viscosities; the viscous force
+        # of particle i then uses mu_i = sum_k volume_frac_ik * mu_k.
+        mus = env.get("phase_dynamic_viscosity")
+        if mus is not None and len(mus) != self.phase_num:
+            raise Exception("phase_dynamic_viscosity must have phase_num entries")
+        self.phase_dynamic_viscosity = None if mus is None else [_f32(v) for v in mus]
         self.fluid_max_part_num = int(read_param(self.scenario_buffer["fluid"]["max_part_num"], "fluid_max_part_num"))
         self.bound_max_part_num = int(read_param(self.scenario_buffer["bound"]["max_part_num"], "bound_max_part_num"))
         hexes = env.get("phase_color_hex", ["0x6F7DBC"] * self.phase_num)
@@ -203,6 +210,10 @@ class Config:
             p.kernel_sig[k] = self.kernel_sig[k]
         p.r2_max = self.r2_max32 if precision == 4 else self.r2_max64
         p.dynamic_viscosity = self.dynamic_viscosity
+        if self.phase_dynamic_viscosity is not None:
+            p.mu_per_phase = 1
+            for k, v in enumerate(self.phase_dynamic_viscosity):
+                p.phase_viscosity[k] = v
         p.surface_tension_gamma = self.surface_tension_gamma
         p.fbm_diffusion = self.fbm_diffusion_term
         p.fbm_convection = self.fbm_convection_term

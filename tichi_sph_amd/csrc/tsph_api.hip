@@ -36,7 +36,7 @@ int solver_begin(const TsphParams&, TsphControl*, int, int*, int, hipStream_t);
 int verify_fast_quotient(const TsphParams&, int*, hipStream_t);
 template <class T>
 int solver_half(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, int, void*, void*,
-                void*, size_t, hipStream_t);
+                const void*, void*, size_t, hipStream_t);
 template <class T>
 int relist(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template <class T>
@@ -309,13 +309,14 @@ int tsph_solver_begin_signal(const TsphParams* p, TsphControl* ctrl, int32_t sol
 
 static int solver_entry(const TsphParams* p, TsphObject* f, TsphObject* b, TsphControl* ctrl,
                         int32_t k, int half, void* ws, size_t wsb, void* stream,
-                        void* tape_v = nullptr, void* tape_k = nullptr) {
+                        void* tape_v = nullptr, void* tape_k = nullptr,
+                        const void* clamp_ref = nullptr) {
   if (int rc = check_params(p)) return rc;
   if (int rc = check_obj(f, "tsph_solver_iteration")) return rc;
   if (int rc = check_obj(b, "tsph_solver_iteration")) return rc;
   TSPH_REQUIRE(k >= 0 && k < TSPH_MAX_ITER_LOG, TSPH_EINVAL,
                "tsph_solver_iteration: iteration %d outside [0, %d)", k, TSPH_MAX_ITER_LOG);
-  return DISPATCH(p, solver_half, *p, f, b, ctrl, k, half, tape_v, tape_k, ws, wsb,
+  return DISPATCH(p, solver_half, *p, f, b, ctrl, k, half, tape_v, tape_k, clamp_ref, ws, wsb,
                   (hipStream_t)stream);
 }
 
@@ -336,6 +337,12 @@ int tsph_solver_psi_taped(const TsphParams* p, TsphObject* f, TsphObject* b, Tsp
                           void* stream) {
   TSPH_REQUIRE(tape_vel_adv && tape_kappa, TSPH_EINVAL, "tsph_solver_psi_taped: null tape");
   return solver_entry(p, f, b, ctrl, k, 1, ws, wsb, stream, tape_vel_adv, tape_kappa);
+}
+
+int tsph_solver_psi_frozen(const TsphParams* p, TsphObject* f, TsphObject* b, TsphControl* ctrl,
+                           int32_t k, const void* kappa_ref, void* ws, size_t wsb, void* stream) {
+  TSPH_REQUIRE(kappa_ref, TSPH_EINVAL, "tsph_solver_psi_frozen: null kappa_ref");
+  return solver_entry(p, f, b, ctrl, k, 1, ws, wsb, stream, nullptr, nullptr, kappa_ref);
 }
 
 int tsph_relist(const TsphParams* p, TsphObject* f, TsphObject* b, TsphControl* ctrl, void* ws,

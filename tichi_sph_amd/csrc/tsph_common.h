@@ -41,6 +41,8 @@ struct KP {
   T div_thr, comp_thr;
   T two_d2;      // 2*(2+dim) of W_lap (sph_util.py:42)
   T wg_a, wg_b;  // fast gradient factors: 6*sig/h^2 and 6*sig/h
+  int mu_pp;     // per-phase viscosity (TsphParams.mu_per_phase)
+  T mu_k[kMaxP];
 };
 
 // A kernel body's block: index and count within its launch, or within its
@@ -63,6 +65,18 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 // Strict arithmetic (reference evaluation order, single rounding per op).
 // ---------------------------------------------------------------------------
 #define TSPH_STRICT _Pragma("clang fp contract(off)")
+
+// The dynamic viscosity of particle i: the global mu (sph_config.py:148), or
+// with the per-phase extension sum_k phi_ik mu_k, k ascending.
+template <class T>
+__device__ __forceinline__ T mu_of(const KP<T>& k, const T* __restrict__ vf, int i) {
+  TSPH_STRICT
+  if (!k.mu_pp) return k.mu;
+  const T* phi = vf + (long)i * k.P;
+  T m = phi[0] * k.mu_k[0];
+  for (int q = 1; q < k.P; ++q) m = m + phi[q] * k.mu_k[q];
+  return m;
+}
 
 template <class T>
 __device__ __forceinline__ T s_sqrt(T x) {

@@ -75,6 +75,8 @@ KP<T> make_kp(const TsphParams& p) {
   k.inv_h = (T)1 / h;
   k.gps = (T)p.part_size[1];
   k.mu = (T)p.dynamic_viscosity;
+  k.mu_pp = p.mu_per_phase ? 1 : 0;
+  for (int q = 0; q < kMaxP; ++q) k.mu_k[q] = (T)p.phase_viscosity[q];
   k.gamma = (T)p.surface_tension_gamma;
   k.fbmD = (T)p.fbm_diffusion;
   k.fbmC = (T)p.fbm_convection;

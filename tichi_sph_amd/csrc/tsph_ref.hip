@@ -194,6 +194,7 @@ __global__ void __launch_bounds__(kBlock)
       if (act) {
         V4<T> p = o.pos[i], vi = o.vel[i], a = o.acce_adv[i];
         T rd = o.rest_density[i];
+        const T mu = mu_of(k, o.vf, i);  // the global mu, or the per-phase extension
         ref_walk(k, g, cs, no.uid, [&](int j) {
           V4<T> q = no.pos[j], vj = no.vel[j];
           T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
@@ -206,9 +207,9 @@ __global__ void __launch_bounds__(kBlock)
             Ax = Ax + (vi.y - vj.y) * dy;
             if (k.dim == 3) Ax = Ax + (vi.z - vj.z) * dz;
             T den = k.eps + r * r;
-            a.x = a.x + s * (inv * dx) * Ax / den * k.mu / rd;
-            a.y = a.y + s * (inv * dy) * Ax / den * k.mu / rd;
-            if (k.dim == 3) a.z = a.z + s * (inv * dz) * Ax / den * k.mu / rd;
+            a.x = a.x + s * (inv * dx) * Ax / den * mu / rd;
+            a.y = a.y + s * (inv * dy) * Ax / den * mu / rd;
+            if (k.dim == 3) a.z = a.z + s * (inv * dz) * Ax / den * mu / rd;
           }
         });
         o.acce_adv[i] = a;

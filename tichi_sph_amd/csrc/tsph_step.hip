@@ -411,6 +411,7 @@ __device__ __forceinline__ void k_solver_psi_body(Blk blk, KP<T> k, int n, int c
                  const int* __restrict__ nbr_nb, T* __restrict__ psi_adv, T* __restrict__ kappa,
                  V4<T>* __restrict__ krec, const V4<T>* __restrict__ a1b,
                  V4<T>* __restrict__ tape_v, T* __restrict__ tape_k,
+                 const T* __restrict__ clamp_ref,
                  double* __restrict__ partial, TsphControl* ctrl) {
   __shared__ double lds[kWaves];
   if (!ctrl->run_u[kk]) return;
@@ -447,7 +448,10 @@ __device__ __forceinline__ void k_solver_psi_body(Blk blk, KP<T> k, int n, int c
     sum += va.x * B.x + va.y * B.y + va.z * B.z;
     const T rp = rest_psi_of(k, rdens, i);
     T psi = (psi_rho[i] - rp) + sum * dt;
-    if (psi < T(0)) psi = T(0);
+    // IPPE_psi_adv_non_negative (sph.py:219-226); with a clamp reference, the
+    // clamp set of a recorded iteration instead (psi clamped exactly where the
+    // recorded kappa is 0): the path whose derivative the adjoint takes
+    if (clamp_ref ? clamp_ref[i] == T(0) : psi < T(0)) psi = T(0);
     psi_adv[i] = psi;
     const T kap = psi / alpha[i];
     kappa[i] = kap;
@@ -473,8 +477,9 @@ __global__ void __launch_bounds__(kBlock)
                  const int* __restrict__ nbr_nb, T* __restrict__ psi_adv, T* __restrict__ kappa,
                  V4<T>* __restrict__ krec, const V4<T>* __restrict__ a1b,
                  V4<T>* __restrict__ tape_v, T* __restrict__ tape_k,
+                 const T* __restrict__ clamp_ref,
                  double* __restrict__ partial, TsphControl* ctrl) {
-  k_solver_psi_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, n, cap, kk, fpos, vadv, psi_rho, rdens, alpha, fstart, bpos, bstart, nbr_f, nbr_b, nbr_nf, nbr_nb, psi_adv, kappa, krec, a1b, tape_v, tape_k, partial, ctrl);
+  k_solver_psi_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, n, cap, kk, fpos, vadv, psi_rho, rdens, alpha, fstart, bpos, bstart, nbr_f, nbr_b, nbr_nf, nbr_nb, psi_adv, kappa, krec, a1b, tape_v, tape_k, clamp_ref, partial, ctrl);
 }
 
 // Jacobi half-step U: IPPE_update_vel_adv (f,f)(f,b) (sph.py:229-244), with
@@ -584,7 +589,7 @@ __global__ void __launch_bounds__(kBlock)
 // the next kernel (k_surface_tension).
 // ---------------------------------------------------------------------------
 template <class T>
-__device__ __forceinline__ void k_advect_body(Blk blk, KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos, const T* __restrict__ vol,
+__device__ __forceinline__ void k_advect_body(Blk blk, KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos, const T* __restrict__ vf,
              const T* __restrict__ rdens, const int* __restrict__ fstart,
              const int* __restrict__ bstart, const V4<T>* __restrict__ bpos,
              const int* __restrict__ nbr_f, const int* __restrict__ nbr_nf,
@@ -598,7 +603,7 @@ __device__ __forceinline__ void k_advect_body(Blk blk, KP<T> k, int n, int cap, 
     if (i >= n) break;
     const V4<T> p = fpos[i];
     const V4<T> vi = vadv[i];
-    const T fvis = k.two_d2 * k.mu / rdens[i];
+    const T fvis = k.two_d2 * mu_of(k, vf, i) / rdens[i];
     T ax = 0, ay = 0, az = 0, nx = 0, ny = 0, nz = 0;  // this lane's share
     auto term = [&](const Rec2<T>& d) {
       const V4<T>& q = d.a;
@@ -637,13 +642,13 @@ __device__ __forceinline__ void k_advect_body(Blk blk, KP<T> k, int n, int cap, 
 
 template <class T>
 __global__ void __launch_bounds__(kBlock)
-    k_advect(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos, const T* __restrict__ vol,
+    k_advect(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos, const T* __restrict__ vf,
              const T* __restrict__ rdens, const int* __restrict__ fstart,
              const int* __restrict__ bstart, const V4<T>* __restrict__ bpos,
              const int* __restrict__ nbr_f, const int* __restrict__ nbr_nf,
              V4<T>* __restrict__ vadv, V4<T>* __restrict__ vel, V4<T>* __restrict__ acce_adv,
              V4<T>* __restrict__ normal) {
-  k_advect_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, n, cap, fpos, vol, rdens, fstart, bstart, bpos, nbr_f, nbr_nf, vadv, vel, acce_adv, normal);
+  k_advect_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, n, cap, fpos, vf, rdens, fstart, bstart, bpos, nbr_f, nbr_nf, vadv, vel, acce_adv, normal);
 }
 
 // Surface tension loop 2 (sph.py:151-166) + SPH_advection_update_vel_adv
@@ -1129,7 +1134,8 @@ int solver_begin(const TsphParams& prm, TsphControl* ctrl, int solve, int* signa
 
 template <class T>
 int solver_half(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl* ctrl, int kk,
-                int half, void* tape_v, void* tape_k, void* ws, size_t wsb, hipStream_t st) {
+                int half, void* tape_v, void* tape_k, const void* clamp_ref, void* ws, size_t wsb,
+                hipStream_t st) {
   KP<T> k = make_kp<T>(prm);
   WsLayout L;
   TSPH_REQUIRE(carve_ws(ws, wsb, f->capacity, prm.node_num, &L), TSPH_ECAPACITY,
@@ -1145,7 +1151,8 @@ int solver_half(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl
                        (const T*)f->rest_density, (const T*)f->alpha, f->cell_start,
                        (const V4<T>*)b->pos, b->cell_start, f->nbr_f, f->nbr_b, f->nbr_nf,
                        f->nbr_nb, (T*)f->psi_adv, (T*)f->kappa, (V4<T>*)L.krec,
-                       (const V4<T>*)L.a1b, (V4<T>*)tape_v, (T*)tape_k, L.partial, ctrl);
+                       (const V4<T>*)L.a1b, (V4<T>*)tape_v, (T*)tape_k, (const T*)clamp_ref,
+                       L.partial, ctrl);
   if (half & 2)
     hipLaunchKernelGGL(k_solver_update<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
                        kk, (const V4<T>*)f->pos, (const T*)f->mass, (const T*)f->kappa,
@@ -1165,7 +1172,7 @@ int advect(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl* ctr
   if (f->n == 0) return 0;
   int nb = blocks_for(f->n);
   hipLaunchKernelGGL(k_advect<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
-                     (const V4<T>*)f->pos, (const T*)f->vol, (const T*)f->rest_density,
+                     (const V4<T>*)f->pos, (const T*)f->volume_frac, (const T*)f->rest_density,
                      f->cell_start, b->cell_start, (const V4<T>*)b->pos, f->nbr_f, f->nbr_nf,
                      (V4<T>*)f->vel_adv, (V4<T>*)f->vel, (V4<T>*)f->acce_adv,
                      (V4<T>*)f->normal);
@@ -1439,7 +1446,7 @@ __global__ void __launch_bounds__(kBlock) kb_solver_psi(const char* __restrict__
                          (const T*)f.alpha, f.cell_start, (const V4<T>*)e.b.pos, e.b.cell_start,
                          f.nbr_f, f.nbr_b, f.nbr_nf, f.nbr_nb, (T*)f.psi_adv, (T*)f.kappa,
                          (V4<T>*)e.L.krec, (const V4<T>*)e.L.a1b, (V4<T>*)nullptr, (T*)nullptr,
-                         e.L.partial, e.ctrl);
+                         (const T*)nullptr, e.L.partial, e.ctrl);
   });
 }
 
@@ -1459,7 +1466,7 @@ template <class T>
 __global__ void __launch_bounds__(kBlock) kb_advect(const char* __restrict__ tab, int ns) {
   for_tiles<T, kTileFluid>(tab, ns, [&](const SceneEntry<T>& e, int, Blk blk) {
     const TsphObject& f = e.f;
-    k_advect_body<T>(blk, e.k, f.n, f.capacity, (const V4<T>*)f.pos, (const T*)f.vol,
+    k_advect_body<T>(blk, e.k, f.n, f.capacity, (const V4<T>*)f.pos, (const T*)f.volume_frac,
                      (const T*)f.rest_density, f.cell_start, e.b.cell_start,
                      (const V4<T>*)e.b.pos, f.nbr_f, f.nbr_nf, (V4<T>*)f.vel_adv, (V4<T>*)f.vel,
                      (V4<T>*)f.acce_adv, (V4<T>*)f.normal);
@@ -1581,8 +1588,8 @@ template int density_alpha<float>(const TsphParams&, TsphObject*, TsphObject*, T
 template int density_alpha<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template int solver_begin<float>(const TsphParams&, TsphControl*, int, int*, int, hipStream_t);
 template int solver_begin<double>(const TsphParams&, TsphControl*, int, int*, int, hipStream_t);
-template int solver_half<float>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, int, void*, void*, void*, size_t, hipStream_t);
-template int solver_half<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, int, void*, void*, void*, size_t, hipStream_t);
+template int solver_half<float>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, int, void*, void*, const void*, void*, size_t, hipStream_t);
+template int solver_half<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, int, void*, void*, const void*, void*, size_t, hipStream_t);
 template int advect<float>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template int advect<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template int fbm_round<float>(const TsphParams&, TsphObject*, TsphControl*, void*, size_t, hipStream_t);

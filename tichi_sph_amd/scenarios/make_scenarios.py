@@ -39,9 +39,11 @@ def _r(x):
     return round(x, 12)
 
 
-def tank(gps, interior, layers, fluid_blocks, phase_rest_density, gravity, mu, max_extra=0):
+def tank(gps, interior, layers, fluid_blocks, phase_rest_density, gravity, mu, max_extra=0,
+         phase_mu=None):
     """interior: lattice points per axis inside the walls; fluid_blocks: list
-    of (start_idx, count, volume_frac) per block, in lattice units."""
+    of (start_idx, count, volume_frac) per block, in lattice units; phase_mu:
+    per-phase dynamic viscosities (the phase_dynamic_viscosity extension)."""
     s = gps
     d = len(interior)
     fluid_objs = []
@@ -67,13 +69,16 @@ def tank(gps, interior, layers, fluid_blocks, phase_rest_density, gravity, mu, m
     lb = [_r(-layers * s - 2 * s) for a in range(d)]
     rt = [_r(ben[a] + 2 * s) for a in range(d)]
     P = len(phase_rest_density)
+    env = {
+        "global_part_size": gps, "global_dynamic_viscosity": mu, "sim_dim": d,
+        "sim_space_lb": lb, "sim_space_rt": rt, "gravity": gravity, "phase_num": P,
+        "phase_rest_density": phase_rest_density,
+        "phase_color_hex": ["0x6F7DBC", "0xFA8012", "0xFA8072"][:P],
+    }
+    if phase_mu is not None:
+        env["phase_dynamic_viscosity"] = phase_mu
     return {
-        "sim_env": {
-            "global_part_size": gps, "global_dynamic_viscosity": mu, "sim_dim": d,
-            "sim_space_lb": lb, "sim_space_rt": rt, "gravity": gravity, "phase_num": P,
-            "phase_rest_density": phase_rest_density,
-            "phase_color_hex": ["0x6F7DBC", "0xFA8012", "0xFA8072"][:P],
-        },
+        "sim_env": env,
         "fluid": {"max_part_num": nf + max_extra, "objs": fluid_objs},
         "bound": {"max_part_num": nb, "objs": [
             {"type": "box", "vel": [0] * d, "particle_relaxing_factor": 1.0, "start_pos": bst,
@@ -92,15 +97,16 @@ def scenes():
     # Small 3D dam break for parity tests: 20^3 fluid in a 40 x 20 x 20 tank
     out["dambreak_3d_8k"] = (tank(0.01, [40, 20, 20], 2, [([0, 0, 0], [20, 20, 20], [1])], [1000],
                                   [0, -9.81, 0], 0.01), solver_config("DFSPH", cs=100))
-    # 3D two-phase VFSPH: 125^3 in a 1 m box, phase A (1000) below, phase B (500) above, r = 0.004
+    # 3D two-phase VFSPH: 125^3 in a 1 m box, phase A (1000, viscosity 0.01) below, phase B
+    # (500, viscosity 0.05) above, r = 0.004
     out["two_phase_3d_2m"] = (tank(0.008, [125, 125, 125], 2,
                                    [([0, 0, 0], [125, 62, 125], [1, 0]), ([0, 62, 0], [125, 63, 125], [0, 1])],
-                                   [1000, 500], [0, -9.81, 0], 0.01),
+                                   [1000, 500], [0, -9.81, 0], 0.01, phase_mu=[0.01, 0.05]),
                               solver_config("VFSPH", cs=80, fbm_d=1e-4, fbm_c=50, gamma=0.07))
     # Small two-phase scene for parity tests: 16^3
     out["two_phase_3d_4k"] = (tank(0.008, [16, 16, 16], 2,
                                    [([0, 0, 0], [16, 8, 16], [1, 0]), ([0, 8, 0], [16, 8, 16], [0, 1])],
-                                   [1000, 500], [0, -9.81, 0], 0.01),
+                                   [1000, 500], [0, -9.81, 0], 0.01, phase_mu=[0.01, 0.05]),
                               solver_config("VFSPH", cs=80, fbm_d=1e-4, fbm_c=50, gamma=0.07))
     # Differentiable dam break: 50 x 80 x 50 = 200,000 fluid in a 1.0 x 1.0 x 0.5 m tank, r = 0.005,
     # starting from mu = 0.01, gamma = 0.07 (the gradients are taken w.r.t. both)

@@ -414,12 +414,14 @@ class Simulation:
         p.compression_threshold = float("inf")
         return p
 
-    def _solve(self, solve, tape=None, force=None):
+    def _solve(self, solve, tape=None, force=None, frozen=None):
         """Run solve 0 (divergence) or 1 (density) to its stop rule.  With
         `tape` (a list), each launched iteration k records vel_adv entering it
         and its kappa into new tensors appended as (vel_adv, kappa); entries
         past the iterations that ran are dropped.  `force` = n runs exactly n
-        iterations."""
+        iterations.  `frozen` (a list of recorded kappa tensors, one per
+        iteration) clamps psi where the recorded kappa is 0 instead of where
+        psi is negative (tsph_solver_psi_frozen)."""
         params = self.params if force is None else self.forced_params(force)
         P = ctypes.byref(params)
         ws, wsb = self._ws()
@@ -439,7 +441,10 @@ class Simulation:
         cap = self.fluid.t["pos"].shape[0]
         while k < kmax:
             for _ in range(min(chunk, kmax - k)):
-                if tape is None:
+                if frozen is not None:
+                    self._call("solver_psi", self.lib.tsph_solver_psi_frozen, P, f, b, self.ctrl_ptr(), k,
+                               ctypes.c_void_p(frozen[k].data_ptr()), ws, wsb, st)
+                elif tape is None:
                     self._call("solver_psi", self.lib.tsph_solver_psi, P, f, b, self.ctrl_ptr(), k, ws, wsb, st)
                 else:
                     tv = torch.empty(cap, 4, dtype=self.fluid.dtype, device=self.device)
