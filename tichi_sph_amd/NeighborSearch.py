@@ -1,6 +1,8 @@
 """The reference's NeighborSearch surface (NeighborSearch.py:5-132).
 
-`NeighborSearch(sim, number_of_uids)` builds, for every particle of each
+`NeighborSearch(number_of_uids)` (the reference's form, on the Config bound
+by tichi_sph_amd.use_config) or `NeighborSearch(sim, number_of_uids)` builds,
+for every particle of each
 object, the candidate list of the reference's fill_neighbs
 (NeighborSearch.py:84-101): every particle of the neighbouring cells, self
 included and unfiltered by distance, grouped by uid, in template order then
@@ -26,10 +28,19 @@ class PartNeighbs:
 
 
 class NeighborSearch:
-    def __init__(self, sim, number_of_uids=2):
-        self.sim = sim
+    def __init__(self, sim=None, number_of_uids=2):
+        if isinstance(sim, int):  # NeighborSearch(number_of_uids), NeighborSearch.py:13
+            sim, number_of_uids = None, sim
+        self._sim = sim
         self.number_of_uids = number_of_uids
         self.parts = [PartNeighbs() for _ in range(number_of_uids)]
+
+    @property
+    def sim(self):
+        if self._sim is None:
+            from . import binding
+            self._sim = binding.simulation()
+        return self._sim
 
     def establish_neighbs(self, *objs):
         sim = self.sim

@@ -7,7 +7,7 @@ Fluid, Ngrid, NeighborSearch, the sph.py kernel functions and sph_step().
 """
 from .config import Config, read_param  # noqa: F401
 from .sph_obj import Fluid, Ngrid  # noqa: F401
-from .solver import (EventTimer, GlobalVariable, Simulation, StepStats, inlet_helper_get_dt,  # noqa: F401
-                     load_simulation, simulation_from_dicts)
+from .solver import (EventTimer, GlobalVariable, Simulation, StepStats, current_simulation,  # noqa: F401
+                     inlet_helper_get_dt, load_simulation, simulation_from_dicts, use_config)
 from .NeighborSearch import NeighborSearch  # noqa: F401
 from .grid import Grid  # noqa: F401

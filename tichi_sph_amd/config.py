@@ -44,6 +44,33 @@ def r2_in_range_max(h, dtype):
     return float(x)
 
 
+class Field0D(float):
+    """A float the reference keeps in a 0-D field: `config.x[None]` reads it
+    (sph_config.py: ti.field(float, ())); otherwise it is a plain float."""
+
+    def __getitem__(self, k):
+        if k is None:
+            return float(self)
+        raise TypeError("a 0-D field is indexed with [None]")
+
+
+class IntField0D(int):
+    """An int the reference keeps in a 0-D field (`config.x[None]`)."""
+
+    def __getitem__(self, k):
+        if k is None:
+            return int(self)
+        raise TypeError("a 0-D field is indexed with [None]")
+
+
+# Config values the reference reads as 0-D fields from the run loop
+# (main.py:9-10, 199-234, 348; sph_config.py:145-194)
+_FIELDS_F = ("divergence_threshold", "compression_threshold", "dt", "cfl_factor", "dynamic_viscosity",
+             "surface_tension_gamma", "fbm_diffusion_term", "fbm_convection_term")
+_FIELDS_I = ("fluid_max_part_num", "bound_max_part_num", "iter_threshold_min", "iter_threshold_max", "cs",
+             "wc_gamma", "node_num", "gui_fps")
+
+
 class Config:
     """Constants of one scene, from config.json + scenario.json."""
 
@@ -70,6 +97,11 @@ class Config:
         self.calculate_kernel_param()
         self.init_neighb_param()
         self.calculate_neighb_param()
+        for name in _FIELDS_F:
+            setattr(self, name, Field0D(getattr(self, name)))
+        for name in _FIELDS_I:
+            if getattr(self, name, None) is not None:
+                setattr(self, name, IntField0D(getattr(self, name)))
 
     @classmethod
     def from_files(cls, config_file, scenario_file):

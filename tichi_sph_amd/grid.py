@@ -13,7 +13,13 @@ import torch
 
 
 class Grid:
-    def __init__(self, sim, literal_shape=False):
+    """`Grid()` (the reference's form, main.py:11) is built on the Simulation
+    of the Config bound by tichi_sph_amd.use_config; `Grid(sim)` on `sim`."""
+
+    def __init__(self, sim=None, literal_shape=False):
+        if sim is None:
+            from . import binding
+            sim = binding.simulation()
         cfg = sim.config
         d = cfg.dim
         lb = np.array(cfg.sim_space_lb[:d], dtype=np.float32)

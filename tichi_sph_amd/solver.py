@@ -19,7 +19,7 @@ import warnings
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, binding
 from .config import Config
 from .sph_obj import Fluid, Ngrid
 
@@ -142,7 +142,8 @@ class Simulation:
     GPU only)."""
 
     def __init__(self, config, device="cuda", precision=4, fbm=False, wcsph=False, cfl=True,
-                 list_cap_f=64, list_cap_b=48, fluid_max=None, bound_max=None, fast_quotient=True):
+                 list_cap_f=64, list_cap_b=48, fluid_max=None, bound_max=None, fast_quotient=True,
+                 fluid=None, bound=None):
         self.config = config
         self.device = torch.device(device)
         self.precision = precision
@@ -156,8 +157,19 @@ class Simulation:
                 self.params.flags |= _lib.TSPH_FLAG_FAST_QUOTIENT
         fm = config.fluid_max_part_num if fluid_max is None else fluid_max
         bm = config.bound_max_part_num if bound_max is None else bound_max
-        self.fluid = Fluid(config, fm, 0, device, precision, list_cap_f, list_cap_b)
-        self.bound = Fluid(config, bm, 1, device, precision, list_cap_f, list_cap_b)
+        # objects built by the caller (the reference's Fluid(max_part_num),
+        # main.py:9-10) are adopted; otherwise the Simulation builds its own
+        for o, u in ((fluid, 0), (bound, 1)):
+            if o is not None and (o.uid != u or o.precision != precision or o.config is not config
+                                  or o.device != self.device):
+                raise ValueError(f"object uid {o.uid} does not fit this Simulation (uid {u}, same Config, "
+                                 "precision and device)")
+        self.fluid = fluid if fluid is not None else Fluid(config, fm, 0, device, precision, list_cap_f,
+                                                           list_cap_b)
+        self.bound = bound if bound is not None else Fluid(config, bm, 1, device, precision, list_cap_f,
+                                                           list_cap_b)
+        fm, bm = self.fluid.max_part_num, self.bound.max_part_num
+        self.fluid.sim = self.bound.sim = self
         cap = max(fm, 1) + max(bm, 1)
         self.ws = torch.zeros(int(self.lib.tsph_workspace_bytes(ctypes.byref(self.params), cap)),
                               dtype=torch.uint8, device=self.device)
@@ -499,7 +511,8 @@ class Simulation:
 
     # -------------------------------------------------- reference-order step
     def sph_step_reference(self, wcsph=None, fbm=None):
-        """main.py:175-256 kernel by kernel through the sph.py surface
+        """main.py:175-256 kernel by kernel through the sph.py surface, in the
+        reference's call forms f(ngrid, obj, nobj) and f(obj)
         (cfl_condition first, as run_step does at main.py:348)."""
         from . import sph as K
 
@@ -509,71 +522,94 @@ class Simulation:
         fluid.cell_table_ok = False  # this path walks the Ngrid, not the cell order
         cfg = self.config
         if self.cfl:
-            K.cfl_condition(self, fluid)
+            K.cfl_condition(fluid)
         ngrid.clear_node()
         ngrid.encode(fluid)
         ngrid.encode(bound)
         ngrid.mem_shift()
         ngrid.fill_node(fluid)
         ngrid.fill_node(bound)
-        K.SPH_clean_value(self, fluid)
-        K.SPH_clean_value(self, bound)
-        K.SPH_prepare_attr(self, fluid, fluid)
-        K.SPH_prepare_attr(self, fluid, bound)
-        K.SPH_prepare_attr(self, bound, bound)
-        K.SPH_prepare_attr(self, bound, fluid)
-        K.SPH_prepare_alpha_1(self, fluid, fluid)
-        K.SPH_prepare_alpha_1(self, fluid, bound)
-        K.SPH_prepare_alpha_2(self, fluid, fluid)
-        K.SPH_prepare_alpha_2(self, bound, fluid)
-        K.SPH_prepare_alpha(self, fluid)
-        K.SPH_prepare_alpha(self, bound)
+        K.SPH_clean_value(fluid)
+        K.SPH_clean_value(bound)
+        K.SPH_prepare_attr(ngrid, fluid, fluid)
+        K.SPH_prepare_attr(ngrid, fluid, bound)
+        K.SPH_prepare_attr(ngrid, bound, bound)
+        K.SPH_prepare_attr(ngrid, bound, fluid)
+        K.SPH_prepare_alpha_1(ngrid, fluid, fluid)
+        K.SPH_prepare_alpha_1(ngrid, fluid, bound)
+        K.SPH_prepare_alpha_2(ngrid, fluid, fluid)
+        K.SPH_prepare_alpha_2(ngrid, bound, fluid)
+        K.SPH_prepare_alpha(fluid)
+        K.SPH_prepare_alpha(bound)
         out = {}
         for solve, thr in ((0, cfg.divergence_threshold), (1, cfg.compression_threshold)):
             it = 0
             errs = []
             if solve == 0:
-                K.SPH_vel_2_vel_adv(self, fluid)
+                K.SPH_vel_2_vel_adv(fluid)
             while it < cfg.iter_threshold_min or self.compression() > thr:
-                K.IPPE_adv_psi_init(self, fluid)
-                K.IPPE_adv_psi(self, fluid, fluid)
-                K.IPPE_adv_psi(self, fluid, bound)
-                K.IPPE_psi_adv_non_negative(self, fluid)
+                K.IPPE_adv_psi_init(fluid)
+                K.IPPE_adv_psi(ngrid, fluid, fluid)
+                K.IPPE_adv_psi(ngrid, fluid, bound)
+                K.IPPE_psi_adv_non_negative(fluid)
                 errs.append(self.compression())
-                K.IPPE_update_vel_adv(self, fluid, fluid)
-                K.IPPE_update_vel_adv(self, fluid, bound)
+                K.IPPE_update_vel_adv(ngrid, fluid, fluid)
+                K.IPPE_update_vel_adv(ngrid, fluid, bound)
                 it += 1
                 if it > cfg.iter_threshold_max:
                     break
             out[solve] = (it, errs)
             if solve == 0:
-                K.SPH_vel_adv_2_vel(self, fluid)
-                K.SPH_advection_gravity_acc(self, fluid)
-                K.SPH_advection_viscosity_acc(self, fluid, fluid)
-                K.SPH_advection_surface_tension_acc(self, fluid, fluid)
-                K.SPH_advection_update_vel_adv(self, fluid)
+                K.SPH_vel_adv_2_vel(fluid)
+                K.SPH_advection_gravity_acc(fluid)
+                K.SPH_advection_viscosity_acc(ngrid, fluid, fluid)
+                K.SPH_advection_surface_tension_acc(ngrid, fluid, fluid)
+                K.SPH_advection_update_vel_adv(fluid)
         if wcsph:
-            K.WC_pressure_val(self, fluid)
-            K.WC_pressure_acce(self, fluid, fluid)
-            K.WC_pressure_acce(self, fluid, bound)
-            K.SPH_advection_update_vel_adv(self, fluid)
+            K.WC_pressure_val(fluid)
+            K.WC_pressure_acce(ngrid, fluid, fluid)
+            K.WC_pressure_acce(ngrid, fluid, bound)
+            K.SPH_advection_update_vel_adv(fluid)
         rounds = 0
         if fbm:
             while self.control().general_flag > 0 and rounds < 1000:
-                K.SPH_FBM_clean_tmp(self, fluid)
-                K.SPH_FBM_convect(self, fluid, fluid)
-                K.SPH_FBM_diffuse(self, fluid, fluid)
-                K.SPH_FBM_check_tmp(self, fluid)
+                K.SPH_FBM_clean_tmp(fluid)
+                K.SPH_FBM_convect(ngrid, fluid, fluid)
+                K.SPH_FBM_diffuse(ngrid, fluid, fluid)
+                K.SPH_FBM_check_tmp(fluid)
                 rounds += 1
-            K.SPH_update_volume_frac(self, fluid)
-        K.SPH_update_mass(self, fluid)
-        K.SPH_update_pos(self, fluid)
-        K.SPH_update_energy(self, fluid)
+            K.SPH_update_volume_frac(fluid)
+        K.SPH_update_mass(fluid)
+        K.SPH_update_pos(fluid)
+        K.SPH_update_energy(fluid)
         return out[0][0], out[1][0], out[0][1], out[1][1], rounds
 
     def compression(self):
         c = self.control().compression
         return float(np.float32(c)) if self.precision == 4 else float(c)
+
+
+def _simulation_of_binding(b):
+    return Simulation(b.config, fluid=b.objs[0], bound=b.objs[1], **b.kw)
+
+
+binding._FACTORY = _simulation_of_binding
+
+
+def use_config(config, **kw):
+    """Bind the reference's module-level construction to `config` (its
+    global config, sph_config.py): after this, `Ngrid()`,
+    `Fluid(max_part_num=...)` (fluid first, then the boundary, main.py:8-10)
+    and `NeighborSearch(number_of_uids)` build the objects the sph.py kernels
+    take, and `current_simulation()` gives the Simulation that owns them
+    (scenario ingestion, the fused step, the writers).  `kw` are the
+    Simulation's options (device, precision, fbm, cfl, ...)."""
+    return binding.use_config(config, **kw)
+
+
+def current_simulation():
+    """The Simulation of the objects built against the bound Config."""
+    return binding.simulation()
 
 
 def load_simulation(config_file, scenario_file, **kw):

@@ -1,15 +1,23 @@
 """The reference's kernel surface (sph.py): one function per Taichi kernel,
-same names and (ngrid, obj, nobj) argument meaning, each enqueuing the
-matching HIP kernel through tsph_ref_kernel.  The grid is the Simulation's
-reference-layout Ngrid (build it with ngrid.clear_node / encode / mem_shift /
-fill_node first, as main.py:178-183 does).
+with the reference's signatures, each enqueuing the matching HIP kernel
+through tsph_ref_kernel:
 
-The first argument is the Simulation that owns the objects (the reference
-reaches the same state through module globals).
+    f(ngrid, obj, nobj)    the neighbour kernels (sph.py:50-97, 114-193, ...)
+    f(obj)                 the per-particle kernels (sph.py:22-47, 100-111, ...)
+
+`ngrid` is the reference-layout grid of the objects' Simulation (build it with
+ngrid.clear_node / encode / mem_shift / fill_node first, as main.py:178-183
+does); the Simulation itself is accepted in its place, and as a leading
+argument of the per-particle kernels, f(sim, obj).
 """
 import ctypes
 
 from . import _lib
+
+
+def _sim_of(x):
+    """The Simulation behind an Ngrid (or a Simulation given directly)."""
+    return x if hasattr(x, "ngrid") and hasattr(x, "params") else x.sim
 
 
 def _run(sim, name, obj, nobj=None):
@@ -25,7 +33,14 @@ def _run(sim, name, obj, nobj=None):
 
 
 def _make_obj_kernel(name):
-    def fn(sim, obj):
+    def fn(*args):
+        if len(args) == 1:  # f(obj), the reference's form
+            obj = args[0]
+            sim = obj.sim
+        elif len(args) == 2:  # f(sim, obj)
+            sim, obj = _sim_of(args[0]), args[1]
+        else:
+            raise TypeError(f"{name}(obj) takes one particle object")
         _run(sim, name, obj)
     fn.__name__ = name
     fn.__doc__ = f"{name}(obj): reference kernel of the same name (sph.py)."
@@ -33,8 +48,8 @@ def _make_obj_kernel(name):
 
 
 def _make_pair_kernel(name):
-    def fn(sim, obj, nobj):
-        _run(sim, name, obj, nobj)
+    def fn(ngrid, obj, nobj):
+        _run(_sim_of(ngrid), name, obj, nobj)
     fn.__name__ = name
     fn.__doc__ = f"{name}(ngrid, obj, nobj): reference kernel of the same name (sph.py)."
     return fn
@@ -57,10 +72,11 @@ for _n in _PAIR:
 __all__ = _OBJ + _PAIR
 
 
-def map_velocity(sim, grid, nobj):
+def map_velocity(ngrid, grid, nobj):
     """map_velocity(ngrid, grid, nobj), sph.py:382-397: SPH interpolation of
     nobj's velocity onto the grid's nodes (grid.vel).  nobj's cell table must
     be this step's (after Simulation.sph_step), like the reference's Ngrid."""
+    sim = _sim_of(ngrid)
     if not getattr(nobj, "cell_table_ok", False):
         raise _lib.TsphError("map_velocity: the object's cell table is stale (run Simulation.sph_step first)")
     shape = (ctypes.c_int32 * 3)(*(list(grid.shape) + [1] * (3 - len(grid.shape))))

@@ -28,18 +28,53 @@ SCALAR_FIELDS = ("mass", "rest_density", "rest_volume", "W", "sph_compression", 
 INT_FIELDS = ("pid", "flag", "cell", "in_cell", "nbr_nf", "nbr_nb")
 
 
-class Fluid:
-    """A particle object (sph_obj.py:8-93): `Fluid(config, max_part_num)`;
-    the first object created in a Simulation is the fluid (uid 0), the second
-    the boundary (uid 1), as in main.py:9-10."""
+class _Scalar0D:
+    """A 0-D device value read as `obj.x[None]`, as the reference reads its
+    0-D fields (e.g. fluid.compression[None], main.py:201, 222)."""
 
-    def __init__(self, config, max_part_num, uid=0, device="cuda", precision=4,
+    def __init__(self, read):
+        self._read = read
+
+    def __getitem__(self, k):
+        if k is not None:
+            raise TypeError("a 0-D field is indexed with [None]")
+        return self._read()
+
+
+class Fluid:
+    """A particle object (sph_obj.py:8-93).  Two forms:
+    * `Fluid(max_part_num=n)`, the reference's (main.py:9-10): the object is
+      built against the Config bound by tichi_sph_amd.use_config and takes the
+      next uid in creation order (sph_obj.py:13-14), fluid first (uid 0),
+      then the boundary (uid 1);
+    * `Fluid(config, max_part_num, uid, device, precision)`, as a Simulation
+      builds its own."""
+
+    def __init__(self, config=None, max_part_num=None, uid=None, device=None, precision=None,
                  list_cap_f=64, list_cap_b=48):
+        if not hasattr(config, "phase_num"):  # the reference's form: Fluid(max_part_num)
+            from . import binding
+            if max_part_num is None:
+                max_part_num = config
+            b = binding.current()
+            if b.sim is not None:
+                raise RuntimeError("the bound Config's Simulation already exists: create the objects first")
+            config = b.config
+            uid = len(b.objs) if uid is None else uid
+            device = b.device if device is None else device
+            precision = b.precision if precision is None else precision
+            b.objs.append(self)
+        if max_part_num is None:
+            raise TypeError("Fluid needs max_part_num")
         self.config = config
         self.max_part_num = int(max_part_num)
-        self.uid = uid
-        self.device = torch.device(device)
-        self.precision = precision
+        self.uid = 0 if uid is None else uid
+        self.device = torch.device("cuda" if device is None else device)
+        self.precision = 4 if precision is None else precision
+        precision = self.precision
+        device = self.device
+        uid = self.uid
+        self.sim = None  # the Simulation that owns the object (set by it)
         self.dtype = torch.float32 if precision == 4 else torch.float64
         self.np_dtype = np.float32 if precision == 4 else np.float64
         self.part_num = 0
@@ -102,6 +137,12 @@ class Fluid:
             if getattr(s, name) != self.t[name].data_ptr():
                 self.t[name], self.t[name + "_b"] = self.t[name + "_b"], self.t[name]
         self._refresh_struct()
+
+    @property
+    def compression(self):
+        """fluid.compression (sph_obj.py), read as compression[None]: the
+        average clamped density error of the last Jacobi iteration."""
+        return _Scalar0D(lambda: self.sim.compression())
 
     # DFSPH/VFSPH aliases, sph_obj.py:65-73
     @property
@@ -273,11 +314,32 @@ class Fluid:
         self.t["pos"][:n, 3] = self.t[self.X_name][:n]
 
 
+class _BoundNgrid:
+    """`Ngrid()` in the reference's form (main.py:8): the grid of the
+    Simulation of the bound Config's objects, resolved on first use (the
+    objects are created after the grid in main.py:8-10)."""
+
+    @property
+    def sim(self):
+        from . import binding
+        return binding.simulation()
+
+    def __getattr__(self, name):
+        return getattr(self.sim.ngrid, name)
+
+
 class Ngrid:
     """The reference's shared uniform grid (sph_obj.py:280-323) over the
     objects' current storage order.  clear_node / encode / mem_shift /
     fill_node keep the reference call sequence; the grid is materialised by
-    one library call when the last encoded object has been filled."""
+    one library call when the last encoded object has been filled.
+    `Ngrid()` (the reference's form) gives the grid of the Config bound by
+    tichi_sph_amd.use_config."""
+
+    def __new__(cls, sim=None):
+        if sim is None:
+            return _BoundNgrid()
+        return super().__new__(cls)
 
     def __init__(self, sim):
         self.sim = sim
