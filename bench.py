@@ -286,10 +286,24 @@ def batched_leg(steps=20, n_scenes=64):
             "autotuned_grid": grid, "max_iterations_last_step": [int(its[0].max()), int(its[1].max())]}
 
 
-def cpu_baseline(steps=2):
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or the platform string."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.lower().startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline(steps=5, warmup=2):
     """The oracle (C++ restatement of the reference step, reference schedule:
     atomic bucket grid each step, unsorted storage, separate (obj, nobj)
-    sweeps) on this host's cores, on a bounded sample of the same workload."""
+    sweeps; g++ -O2 -fopenmp, no fast-math) on this host's cores, on a
+    bounded sample of the same workload: `warmup` untimed steps, then
+    `steps` timed ones (about 10-15 s of host work at 1M particles)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     cfg = json.load(open(os.path.join(ROOT, "tichi_sph_amd", "scenarios", WORKLOAD + ".config.json")))
@@ -302,7 +316,8 @@ def cpu_baseline(steps=2):
     O.lib().oracle_set_threads(cores)
     o = O.Oracle(cfg, scn, 4)
     o.init_scenario()
-    o.step(O.STEP_CFL)  # warm-up
+    for _ in range(warmup):
+        o.step(O.STEP_CFL)
     t0 = time.perf_counter()
     its = []
     for _ in range(steps):
@@ -312,7 +327,8 @@ def cpu_baseline(steps=2):
     O.lib().oracle_set_threads(1)
     nf = o.n[0]
     return {"value": nf * steps / t, "unit": "particle-steps/s", "cores": cores, "kind": "port",
-            "sample": f"{WORKLOAD}: {steps} steps after 1 warm-up step ({nf} fluid particles), "
+            "cpu": cpu_model(),
+            "sample": f"{WORKLOAD}: {steps} steps after {warmup} warm-up steps ({nf} fluid particles), "
                       f"{t:.1f} s wall, iterations {its}"}
 
 
