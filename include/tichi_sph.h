@@ -456,6 +456,14 @@ int tsph_batch_sort(const TsphBatchInfo* info, void* table, TsphObject* fluid, i
  * grouped launch (0: one block per tile); the result does not depend on it. */
 int tsph_batch_density_alpha(const TsphBatchInfo* info, void* table, int32_t grid, void* stream);
 int tsph_batch_solver_begin(const TsphBatchInfo* info, void* table, int32_t solve, void* stream);
+/* tsph_batch_solver_begin with each scene's early stop signal (as
+ * tsph_solver_begin_signal, no reference counterpart): scene s's update
+ * half-steps store (token << 32) | (2 * it + stop) into signal[s], an array
+ * of nscenes uint64 in pinned, device-mapped host memory (tsph_signal_alloc),
+ * so that the host learns when every scene's solve has ended without
+ * draining the stream. */
+int tsph_batch_solver_begin_signal(const TsphBatchInfo* info, void* table, int32_t solve,
+                                   uint64_t* signal, int32_t token, void* stream);
 int tsph_batch_solver_psi(const TsphBatchInfo* info, void* table, int32_t k, int32_t grid,
                           void* stream);
 int tsph_batch_solver_update(const TsphBatchInfo* info, void* table, int32_t k, int32_t grid,

@@ -110,7 +110,8 @@ EXPORTED = [
     "tsph_ref_kernel", "tsph_solver_psi_taped", "tsph_solver_psi_frozen", "tsph_relist", "tsph_adj_begin",
     "tsph_adj_solver_iteration", "tsph_adj_advect", "tsph_adj_density_alpha", "tsph_adj_end",
     "tsph_batch_table_bytes", "tsph_batch_prepare", "tsph_batch_sort", "tsph_batch_density_alpha",
-    "tsph_batch_solver_begin", "tsph_batch_solver_psi", "tsph_batch_solver_update",
+    "tsph_batch_solver_begin", "tsph_batch_solver_begin_signal", "tsph_batch_solver_psi",
+    "tsph_batch_solver_update",
     "tsph_batch_advect", "tsph_batch_tail", "tsph_batch_stop_state", "tsph_map_velocity",
     "tsph_wcsph", "tsph_sort_object_static", "tsph_sort_static_scratch_bytes",
     "tsph_sort_static_max_particles", "tsph_sort_static_max_cells",
@@ -181,6 +182,7 @@ def load(path=None):
         "tsph_batch_sort": [BI, _vp, O, _i32, _i32, _vp],
         "tsph_batch_density_alpha": [BI, _vp, _i32, _vp],
         "tsph_batch_solver_begin": [BI, _vp, _i32, _vp],
+        "tsph_batch_solver_begin_signal": [BI, _vp, _i32, _vp, _i32, _vp],
         "tsph_batch_solver_psi": [BI, _vp, _i32, _i32, _vp],
         "tsph_batch_solver_update": [BI, _vp, _i32, _i32, _vp],
         "tsph_batch_advect": [BI, _vp, _i32, _vp],

@@ -65,6 +65,15 @@ class SceneBatch:
         self._prepare()
         self._stop_dev = torch.zeros(2 * ns, dtype=torch.int32, device=self.device)
         self._stop_host = torch.zeros(2 * ns, dtype=torch.int32).pin_memory()
+        # early stop signals of the solves: one uint64 per scene in pinned,
+        # device-mapped host memory (tsph_batch_solver_begin_signal)
+        self._token = 0
+        self._sig_host = None
+        if self.device.type == "cuda":
+            h, d = ctypes.c_void_p(), ctypes.c_void_p()
+            _lib.check(self.lib.tsph_signal_alloc(8 * ns, ctypes.byref(h), ctypes.byref(d)), "tsph_signal_alloc")
+            self._sig_host, self._sig_dev = h, d
+            self._sig_np = np.ctypeslib.as_array(ctypes.cast(h, ctypes.POINTER(ctypes.c_uint64)), shape=(ns,))
         self.kmax = max(s.config.iter_threshold_max for s in sims) + 1
         self.pred = [max(max(s.config.iter_threshold_min, 1) for s in sims)] * 2
         self.grid = {g: 0 for g in GROUPS}
@@ -97,6 +106,13 @@ class SceneBatch:
             s._sort_bound()
         self._prepare()
 
+    def __del__(self):
+        import sys
+        h = getattr(self, "_sig_host", None)
+        if h is not None and not sys.is_finalizing():
+            self.lib.tsph_signal_free(h)
+            self._sig_host = None
+
     def _stream(self):
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
@@ -114,13 +130,10 @@ class SceneBatch:
         self.timer.records.append((name, e0, e1))
 
     def _adopt(self):
-        """Make every Fluid's tensors follow the pointers of its descriptor."""
-        for s, sim in enumerate(self.sims):
-            st = sim.fluid._struct
-            for name in _SWAPPED:
-                setattr(st, name, getattr(self._fl[s], name))
-                setattr(st, name + "_b", getattr(self._fl[s], name + "_b"))
-            sim.fluid.adopt_swaps()
+        """The batched sort swapped every scene's front/back state once: each
+        Fluid mirrors that (its tensors and descriptor) when next used."""
+        for sim in self.sims:
+            sim.fluid.note_device_swap()
 
     def _swap_host(self):
         """The sort's front/back swap applied to the host descriptors."""
@@ -150,9 +163,38 @@ class SceneBatch:
         self._call("density_alpha", self.lib.tsph_batch_density_alpha, ctypes.byref(self.info), self._tab(),
                    self.grid["density_alpha"], self._stream())
 
-    def _begin(self, solve):
+    def _begin(self, solve, signal=False):
+        if signal and self._sig_host is not None:
+            self._token = self._token % 0x7FFFFFFF + 1
+            _lib.check(self.lib.tsph_batch_solver_begin_signal(ctypes.byref(self.info), self._tab(), solve,
+                                                               self._sig_dev, self._token, self._stream()),
+                       "tsph_batch_solver_begin_signal")
+            return True
         _lib.check(self.lib.tsph_batch_solver_begin(ctypes.byref(self.info), self._tab(), solve,
                                                     self._stream()), "tsph_batch_solver_begin")
+        return False
+
+    def _wait_signals(self, launched):
+        """(it, stop) of every scene once each scene's update half-steps have
+        decided on the last of the `launched` iterations (or stopped before
+        it), read from the scenes' early signals without draining the
+        stream (a scene whose solve enters no iteration signals (0, stop) from
+        its begin); falls back to the device stop states once the stream has
+        drained."""
+        a = self._sig_np
+        tok = np.uint64(self._token)
+        stream = torch.cuda.current_stream(self.device)
+        spins = 0
+        while True:
+            v = a.copy()
+            w = v & np.uint64(0xFFFFFFFF)
+            stop = (w & np.uint64(1)).astype(np.int64)
+            it = (w >> np.uint64(1)).astype(np.int64)
+            if np.all(((v >> np.uint64(32)) == tok) & ((stop == 1) | (it >= launched))):
+                return it, stop
+            spins += 1
+            if spins % 64 == 0 and stream.query():
+                return self.stop_state()
 
     def _iteration(self, k):
         bi, st = ctypes.byref(self.info), self._stream()
@@ -168,14 +210,16 @@ class SceneBatch:
         self._call("tail", self.lib.tsph_batch_tail, ctypes.byref(self.info), self._tab(),
                    self.grid["tail"], self._stream())
 
-    def _finish_solve(self, solve, k):
+    def _finish_solve(self, solve, k, signal=False):
         """Iterations beyond the k launched ones, one at a time, until every
-        scene's stop rule has ended its solve."""
-        its, stops = self.stop_state()
+        scene's stop rule has ended its solve (read from the early signals
+        when the solve was opened with them, else from the device)."""
+        state = (lambda: self._wait_signals(k)) if signal else self.stop_state
+        its, stops = state()
         while not stops.all() and k < self.kmax:
             self._iteration(k)
             k += 1
-            its, stops = self.stop_state()
+            its, stops = state()
         self.pred[solve] = max(int(its.max()), 1)
         return its
 
@@ -189,11 +233,11 @@ class SceneBatch:
         self._density_alpha()
         its = []
         for solve in (0, 1):
-            self._begin(solve)
+            sig = self._begin(solve, signal=True)
             p = min(self.pred[solve], self.kmax)
             for k in range(p):
                 self._iteration(k)
-            its.append(self._finish_solve(solve, p))
+            its.append(self._finish_solve(solve, p, signal=sig))
             if solve == 0:
                 self._advect()
         self._tail()

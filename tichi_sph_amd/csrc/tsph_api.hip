@@ -48,7 +48,7 @@ int batch_sort(const TsphBatchInfo&, char*, TsphObject*, int, int, hipStream_t);
 template <class T>
 int batch_density_alpha(const TsphBatchInfo&, char*, int, hipStream_t);
 template <class T>
-int batch_solver_begin(const TsphBatchInfo&, char*, int, hipStream_t);
+int batch_solver_begin(const TsphBatchInfo&, char*, int, uint64_t*, int, hipStream_t);
 template <class T>
 int batch_solver_half(const TsphBatchInfo&, char*, int, int, int, hipStream_t);
 template <class T>
@@ -486,7 +486,14 @@ int tsph_batch_density_alpha(const TsphBatchInfo* bi, void* table, int32_t grid,
 }
 int tsph_batch_solver_begin(const TsphBatchInfo* bi, void* table, int32_t solve, void* stream) {
   if (int rc = check_batch(bi, table, "tsph_batch_solver_begin")) return rc;
-  return BDISPATCH(bi, batch_solver_begin, *bi, (char*)table, solve, (hipStream_t)stream);
+  return BDISPATCH(bi, batch_solver_begin, *bi, (char*)table, solve, (uint64_t*)nullptr, 0,
+                   (hipStream_t)stream);
+}
+int tsph_batch_solver_begin_signal(const TsphBatchInfo* bi, void* table, int32_t solve,
+                                   uint64_t* signal, int32_t token, void* stream) {
+  if (int rc = check_batch(bi, table, "tsph_batch_solver_begin_signal")) return rc;
+  return BDISPATCH(bi, batch_solver_begin, *bi, (char*)table, solve, signal, (int)token,
+                   (hipStream_t)stream);
 }
 int tsph_batch_solver_psi(const TsphBatchInfo* bi, void* table, int32_t k, int32_t grid,
                           void* stream) {

@@ -392,6 +392,10 @@ __device__ __forceinline__ void solver_begin_body(const KP<T>& k, TsphControl* c
     else ctrl->incom_iter = 0;
     ctrl->signal = (uint64_t)(uintptr_t)signal;
     ctrl->token = token;
+    if (signal && !enter) {  // a solve that runs no iteration stops here: (it 0, stop)
+      const uint64_t w = ((uint64_t)(uint32_t)token << 32) | 1u;
+      __hip_atomic_store((uint64_t*)signal, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
   }
 }
 template <class T>
@@ -1432,9 +1436,12 @@ __global__ void __launch_bounds__(kBlock) kb_pass_a_bound(const char* __restrict
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) kb_solver_begin(const char* __restrict__ tab, int ns, int solve) {
-  for_tiles<T, kTileScene>(tab, ns,
-                           [&](const SceneEntry<T>& e, int, Blk) { solver_begin_body(e.k, e.ctrl, solve); });
+__global__ void __launch_bounds__(kBlock)
+    kb_solver_begin(const char* __restrict__ tab, int ns, int solve, uint64_t* signal, int token) {
+  // scene s's early stop signal goes to signal[s] (no signal when null)
+  for_tiles<T, kTileScene>(tab, ns, [&](const SceneEntry<T>& e, int s, Blk) {
+    solver_begin_body(e.k, e.ctrl, solve, signal ? (int*)(signal + s) : nullptr, token);
+  });
 }
 
 template <class T>
@@ -1525,9 +1532,10 @@ int batch_density_alpha(const TsphBatchInfo& bi, char* tab, int grid, hipStream_
 }
 
 template <class T>
-int batch_solver_begin(const TsphBatchInfo& bi, char* tab, int solve, hipStream_t st) {
+int batch_solver_begin(const TsphBatchInfo& bi, char* tab, int solve, uint64_t* signal, int token,
+                       hipStream_t st) {
   hipLaunchKernelGGL(kb_solver_begin<T>, dim3(bi.nscenes), dim3(kBlock), 0, st, tab, bi.nscenes,
-                     solve);
+                     solve, signal, token);
   TSPH_LAUNCH_CHECK("batch_solver_begin");
   return 0;
 }
@@ -1573,7 +1581,7 @@ int batch_stop_state(const TsphBatchInfo& bi, char* tab, int* out, hipStream_t s
 
 #define TSPH_BATCH_INST(T)                                                                  \
   template int batch_density_alpha<T>(const TsphBatchInfo&, char*, int, hipStream_t);      \
-  template int batch_solver_begin<T>(const TsphBatchInfo&, char*, int, hipStream_t);       \
+  template int batch_solver_begin<T>(const TsphBatchInfo&, char*, int, uint64_t*, int, hipStream_t); \
   template int batch_solver_half<T>(const TsphBatchInfo&, char*, int, int, int, hipStream_t); \
   template int batch_advect<T>(const TsphBatchInfo&, char*, int, hipStream_t);             \
   template int batch_tail<T>(const TsphBatchInfo&, char*, int, hipStream_t);               \

@@ -261,9 +261,9 @@ class Simulation:
     def _wait_signal(self, token, launched):
         """(it, stop) of the running solve once its update half-steps have
         decided on the last of the `launched` iterations, read from the early
-        signal without draining the stream; falls back to the control block
-        once the stream has drained (a solve that ran no iteration writes no
-        signal)."""
+        signal without draining the stream (a solve that enters no iteration
+        signals (0, stop) from its begin); falls back to the control block
+        once the stream has drained."""
         a = self._signal_np
         stream = torch.cuda.current_stream(self.device)
         spins = 0

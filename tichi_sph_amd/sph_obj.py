@@ -85,7 +85,8 @@ class Fluid:
         P = config.phase_num
         z = lambda *s: torch.zeros(*s, dtype=self.dtype, device=self.device)  # noqa: E731
         zi = lambda *s: torch.zeros(*s, dtype=torch.int32, device=self.device)  # noqa: E731
-        self.t = {}
+        self._t = {}
+        self._pending_swaps = 0  # front/back swaps done on the device, not yet mirrored here
         for name in VEC_FIELDS:
             self.t[name] = z(cap, 4)
         for name in SCALAR_FIELDS:
@@ -114,6 +115,32 @@ class Fluid:
         self.cell_table_ok = False  # cell_start matches the storage (set by the sort)
 
     # ------------------------------------------------------------------ layout
+    _SWAPPED = ("pos", "vel", "mass", "rest_density", "rest_volume", "volume_frac", "pid")
+
+    @property
+    def t(self):
+        """The object's device tensors by field name (front buffers current)."""
+        if self._pending_swaps:
+            self._resolve_swaps()
+        return self._t
+
+    def note_device_swap(self):
+        """A batched sort swapped this object's front/back state on the
+        device (and in the batch's table): mirror it here when next used."""
+        self._pending_swaps += 1
+        self.cell_table_ok = True
+
+    def _resolve_swaps(self):
+        if self._pending_swaps & 1:
+            t, s = self._t, self._struct
+            for name in self._SWAPPED:
+                nb = name + "_b"
+                t[name], t[nb] = t[nb], t[name]
+                a, b = getattr(s, name), getattr(s, nb)
+                setattr(s, name, b)
+                setattr(s, nb, a)
+        self._pending_swaps = 0
+
     def _refresh_struct(self):
         s = self._struct
         s.n = self.part_num
@@ -124,6 +151,8 @@ class Fluid:
             setattr(s, name, t.data_ptr() if t is not None else None)
 
     def struct(self):
+        if self._pending_swaps:
+            self._resolve_swaps()
         self._struct.n = self.part_num
         return self._struct
 
