@@ -174,7 +174,7 @@ class SceneBatch:
                                                     self._stream()), "tsph_batch_solver_begin")
         return False
 
-    def _wait_signals(self, launched):
+    def _wait_signals(self, launched, token=None):
         """(it, stop) of every scene once each scene's update half-steps have
         decided on the last of the `launched` iterations (or stopped before
         it), read from the scenes' early signals without draining the
@@ -182,7 +182,7 @@ class SceneBatch:
         its begin); falls back to the device stop states once the stream has
         drained."""
         a = self._sig_np
-        tok = np.uint64(self._token)
+        tok = np.uint64(self._token if token is None else token)
         stream = torch.cuda.current_stream(self.device)
         spins = 0
         while True:
@@ -210,11 +210,11 @@ class SceneBatch:
         self._call("tail", self.lib.tsph_batch_tail, ctypes.byref(self.info), self._tab(),
                    self.grid["tail"], self._stream())
 
-    def _finish_solve(self, solve, k, signal=False):
+    def _finish_solve(self, solve, k, signal=False, token=None):
         """Iterations beyond the k launched ones, one at a time, until every
         scene's stop rule has ended its solve (read from the early signals
         when the solve was opened with them, else from the device)."""
-        state = (lambda: self._wait_signals(k)) if signal else self.stop_state
+        state = (lambda: self._wait_signals(k, token)) if signal else self.stop_state
         its, stops = state()
         while not stops.all() and k < self.kmax:
             self._iteration(k)
@@ -250,19 +250,21 @@ class SceneBatch:
         iteration predictions; later steps replay them."""
         p0, p1 = (min(p, self.kmax) for p in self.pred)
         torch.cuda.synchronize(self.device)
-        graphs = []
+        graphs, sig, tokens = [], [], []
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
             self._sort(host=False)
             self._density_alpha()
-            self._begin(0)
+            sig.append(self._begin(0, signal=True))
+            tokens.append(self._token)
             for k in range(p0):
                 self._iteration(k)
         graphs.append(g)
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
             self._advect()
-            self._begin(1)
+            sig.append(self._begin(1, signal=True))
+            tokens.append(self._token)
             for k in range(p1):
                 self._iteration(k)
         graphs.append(g)
@@ -270,18 +272,22 @@ class SceneBatch:
         with torch.cuda.graph(g):
             self._tail()
         graphs.append(g)
-        self.graphs = (graphs, p0, p1)
+        # each replay's solves signal with the two tokens recorded here; every
+        # scene's word is rewritten by each solve (by its begin when it
+        # enters no iteration), so a word still holding the other solve's
+        # token is stale
+        self.graphs = (graphs, p0, p1, sig, tokens)
 
     def release_graphs(self):
         self.graphs = None
 
     def _step_graph(self):
-        (ga, gb, gc), p0, p1 = self.graphs
+        (ga, gb, gc), p0, p1, sig, tokens = self.graphs
         ga.replay()
         self._swap_host()
-        it0 = self._finish_solve(0, p0)
+        it0 = self._finish_solve(0, p0, sig[0], tokens[0])
         gb.replay()
-        it1 = self._finish_solve(1, p1)
+        it1 = self._finish_solve(1, p1, sig[1], tokens[1])
         gc.replay()
         self._adopt()
         self.iter_log.append((it0, it1))
