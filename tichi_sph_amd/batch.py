@@ -20,6 +20,7 @@ the number of blocks of the grouped launch (a grid-stride loop over the
 tiles) by timing candidates on the live batch and restoring its state.
 """
 import ctypes
+import sys
 
 import numpy as np
 import torch
@@ -107,7 +108,6 @@ class SceneBatch:
         self._prepare()
 
     def __del__(self):
-        import sys
         h = getattr(self, "_sig_host", None)
         if h is not None and not sys.is_finalizing():
             self.lib.tsph_signal_free(h)
