@@ -453,7 +453,11 @@ int tsph_batch_sort(const TsphBatchInfo* info, void* table, TsphObject* fluid, i
                     int32_t grid, void* stream);
 /* tsph_density_alpha, tsph_solver_begin, _psi, _update, tsph_advect and
  * tsph_tail (fbm = 0) for every scene.  `grid` = number of blocks of the
- * grouped launch (0: one block per tile); the result does not depend on it. */
+ * grouped launch (0: one block per tile); the result does not depend on it.
+ * tsph_batch_density_alpha sets TSPH_OBJ_STATIC_SUMS in the table's boundary
+ * descriptors, as tsph_density_alpha does in the caller's: a binding that
+ * then writes a boundary's positions or masses prepares the table again from
+ * descriptors with the flag cleared. */
 int tsph_batch_density_alpha(const TsphBatchInfo* info, void* table, int32_t grid, void* stream);
 int tsph_batch_solver_begin(const TsphBatchInfo* info, void* table, int32_t solve, void* stream);
 /* tsph_batch_solver_begin with each scene's early stop signal (as

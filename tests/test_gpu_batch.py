@@ -54,6 +54,36 @@ def test_batched_step_bit_identical_to_single_scenes():
     print(f"  iterations per scene: {list(zip(di.tolist(), ii.tolist()))}", flush=True)
 
 
+def test_batched_kept_boundary_sums_follow_boundary_writes():
+    """The grouped pass A keeps each static boundary's (b,b) density sums
+    after its first step (the table's TSPH_OBJ_STATIC_SUMS).  Writing one
+    scene's boundary masses afterwards must make the batch sum them again:
+    the boundary densities and the fluid states stay bit-identical to the
+    scenes' own steps, which see the same write."""
+    from tichi_sph_amd.batch import SceneBatch
+    d = scene_dicts()
+    print(f"[case] kept boundary sums in a batch of {len(d)} scenes, then a boundary mass write", flush=True)
+    ref = make_sims(d)
+    batch = SceneBatch(make_sims(d))
+    before = None
+    for step in range(4):
+        if step == 2:
+            before = batch.sims[0].bound.to_numpy("sph_density")
+            for sim in (batch.sims[0], ref[0]):
+                m = sim.bound.to_numpy("mass")
+                sim.bound.from_numpy("mass", (m * np.float32(1.01)).astype(m.dtype))
+        batch.step()
+        for sim in ref:
+            sim.sph_step()
+        assert_same(batch.sims, ref, f"step {step}")
+        for s, (a, b) in enumerate(zip(batch.sims, ref)):
+            assert np.array_equal(a.bound.to_numpy("sph_density"), b.bound.to_numpy("sph_density")), (step, s)
+    after = batch.sims[0].bound.to_numpy("sph_density")
+    print(f"  boundary density of scene 0 before / after the write: {before.mean():.6g} / {after.mean():.6g}",
+          flush=True)
+    assert not np.array_equal(before, after)
+
+
 def test_graph_replay_matches_eager_batch():
     from tichi_sph_amd.batch import SceneBatch
     d = scene_dicts()

@@ -88,6 +88,7 @@ class SceneBatch:
         ns = self.ns
         self._params = (_lib.TsphParams * ns)(*[s.params for s in self.sims])
         self._bo = (_lib.TsphObject * ns)(*[s.bound.struct() for s in self.sims])
+        self._epochs = [s.bound.sums_epoch for s in self.sims]
         _lib.check(self.lib.tsph_batch_prepare(
             self._params, self._fl, self._bo, self._ctrl, self._ws, self._wsb, ns,
             ctypes.c_void_p(self._staging.data_ptr()), ctypes.c_void_p(self.table.data_ptr()),
@@ -97,9 +98,13 @@ class SceneBatch:
     def _refresh_bounds(self):
         """A scene whose boundary changed since the table was built (new or
         moved boundary particles) is re-sorted and the table rebuilt, so no
-        batched step searches a stale boundary cell order."""
+        batched step searches a stale boundary cell order.  The table is also
+        rebuilt when a boundary's kept (b,b) sums were invalidated (written
+        positions or masses): the grouped pass A marks the table's copy of
+        the flag, which only a rebuild from the host descriptors clears."""
         stale = [s for s in self.sims if s.bound_needs_sort()]
-        if not stale:
+        epochs = any(s.bound.sums_epoch != e for s, e in zip(self.sims, self._epochs))
+        if not stale and not epochs:
             return
         if self.graphs is not None:
             raise RuntimeError("a scene's boundary changed after capture(): release_graphs() first")

@@ -1404,6 +1404,19 @@ __global__ void __launch_bounds__(kBlock) kb_step_reset(const char* __restrict__
   for_tiles<T, kTileScene>(tab, ns, [&](const SceneEntry<T>& e, int, Blk) { step_reset_body(e.ctrl); });
 }
 
+// After the grouped boundary pass A: every scene whose boundary has a kept-sum
+// buffer now holds its (b,b) sums, so later steps read them (the table's copy
+// of TSPH_OBJ_STATIC_SUMS, as tsph_density_alpha sets it in the caller's
+// descriptor).  One thread per scene, a vector store into the table; the
+// grouped kernels read the table at their next launch.
+template <class T>
+__global__ void kb_mark_static_sums(char* tab, int ns) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  TsphObject& b = ((SceneEntry<T>*)tab)[s].b;
+  if (b.static_sums != nullptr && b.n > 0) b.flags |= TSPH_OBJ_STATIC_SUMS;
+}
+
 template <class T>
 __global__ void __launch_bounds__(kBlock) kb_pass_a_fluid(const char* __restrict__ tab, int ns) {
   for_tiles<T, kTileFluid>(tab, ns, [&](const SceneEntry<T>& e, int, Blk blk) {
@@ -1525,8 +1538,10 @@ int batch_density_alpha(const TsphBatchInfo& bi, char* tab, int grid, hipStream_
   hipLaunchKernelGGL(kb_step_reset<T>, dim3(ns), dim3(kBlock), 0, st, tab, ns);
   if (int g = grid_for(bi.tiles[kTileFluid], grid))
     hipLaunchKernelGGL(kb_pass_a_fluid<T>, dim3(g), dim3(kBlock), 0, st, tab, ns);
-  if (int g = grid_for(bi.tiles[kTileBound], grid))
+  if (int g = grid_for(bi.tiles[kTileBound], grid)) {
     hipLaunchKernelGGL(kb_pass_a_bound<T>, dim3(g), dim3(kBlock), 0, st, tab, ns);
+    hipLaunchKernelGGL(kb_mark_static_sums<T>, dim3((ns + 63) / 64), dim3(64), 0, st, tab, ns);
+  }
   TSPH_LAUNCH_CHECK("batch_density_alpha");
   return 0;
 }

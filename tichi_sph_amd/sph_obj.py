@@ -87,6 +87,7 @@ class Fluid:
         zi = lambda *s: torch.zeros(*s, dtype=torch.int32, device=self.device)  # noqa: E731
         self._t = {}
         self._pending_swaps = 0  # front/back swaps done on the device, not yet mirrored here
+        self.sums_epoch = 0  # invalidations of the kept boundary sums (invalidate_static_sums)
         for name in VEC_FIELDS:
             self.t[name] = z(cap, 4)
         for name in SCALAR_FIELDS:
@@ -211,8 +212,10 @@ class Fluid:
 
     def invalidate_static_sums(self):
         """The kept own-object density sums no longer match the particles:
-        the next pass A sums them again."""
+        the next pass A sums them again.  `sums_epoch` counts these events,
+        so a SceneBatch holding a device copy of the flag rebuilds it."""
         self._struct.flags &= ~_lib.TSPH_OBJ_STATIC_SUMS
+        self.sums_epoch += 1
 
     def from_numpy(self, name, arr):
         """Write field `name` from an array in reference particle order.
