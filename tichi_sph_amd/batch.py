@@ -42,11 +42,14 @@ class SceneBatch:
                 raise ValueError("all scenes of a batch must share one precision")
             if s.fbm:
                 raise ValueError("FBM rounds are not batched")
+            if s.wcsph:
+                raise ValueError("the WCSPH block (main.py:236-240) is not batched")
             if s.fluid.part_num == 0:
                 # the grouped stop rule runs in the scene's first fluid tile
                 raise ValueError("every scene of a batch needs fluid particles")
             if s.device != sims[0].device:
                 raise ValueError("all scenes of a batch must live on one device")
+        for s in sims:
             if s.bound_needs_sort():
                 s._sort_bound()
         self.sims = list(sims)
