@@ -54,6 +54,22 @@ def test_batched_step_bit_identical_to_single_scenes():
     print(f"  iterations per scene: {list(zip(di.tolist(), ii.tolist()))}", flush=True)
 
 
+def test_batched_fp64_step_bit_identical_to_single_scenes():
+    """The grouped kernels' FP64 instantiation (the precision of the
+    differentiable runs) against the FP64 single-scene steps."""
+    from tichi_sph_amd.batch import SceneBatch
+    d = scene_dicts()[-3:]
+    print(f"[case] FP64 batch of {len(d)} scenes: {[n for n, _, _ in d]}", flush=True)
+    ref = make_sims(d, precision=8)
+    batch = SceneBatch(make_sims(d, precision=8))
+    for step in range(3):
+        di, ii = batch.step()
+        for s, sim in enumerate(ref):
+            st = sim.sph_step(stats=True)
+            assert (int(di[s]), int(ii[s])) == (st.div_iter, st.incom_iter), (step, s)
+        assert_same(batch.sims, ref, f"fp64 step {step}")
+
+
 def test_batched_kept_boundary_sums_follow_boundary_writes():
     """The grouped pass A keeps each static boundary's (b,b) density sums
     after its first step (the table's TSPH_OBJ_STATIC_SUMS).  Writing one
