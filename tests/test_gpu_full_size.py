@@ -4,8 +4,8 @@
   particles): one fused step against the FP32 oracle, which runs the same
   step on the host's cores: cell keys and the sort order bit-exact, W and the
   densities bit-exact, every other field within the single-step bar, and the
-  total momentum of the step against the oracle's and against the scene's
-  mirror symmetry.
+  total momentum of the step against the oracle's (the scene's mirror
+  symmetry in z is checked after ten steps in test_gpu_parity.py).
 * The 64 batched dam breaks (10k-200k fluid particles each, drawn from a
   seed): two batched steps bit-identical, scene by scene, to the scenes' own
   sph_step(), and the three smallest scenes against the oracle.
@@ -59,8 +59,8 @@ def test_dambreak_1m_step_matches_oracle():
             assert np.array_equal(fl.to_numpy(name), o.get(obj, name)), (obj, name)
     compare_fields(sim, o, FIELDS_F + ["alpha_1", "alpha_2", "normal"], label="1m")
     compare_fields(sim, o, ["alpha"], obj=1, label="1m")
-    # total momentum: equal to the oracle's, and the block spans the tank's
-    # full z extent, so the z walls' pushes cancel (mirror symmetry z -> 1 - z)
+    # total momentum equal to the oracle's (from rest the first step is
+    # gravity alone: the pressure solve's psi clamps to 0)
     m = f.to_numpy("mass").astype(np.float64)
     v = f.to_numpy("vel").astype(np.float64)
     vo = o.get(0, "vel").astype(np.float64)
@@ -68,7 +68,6 @@ def test_dambreak_1m_step_matches_oracle():
     scale = (m[:, None] * np.abs(vo)).sum(0)
     print(f"  momentum gpu {P} oracle {Po} (sum m|v| {scale})", flush=True)
     assert np.all(np.abs(P - Po) <= 1e-5 * scale.max())
-    assert abs(P[2]) <= 1e-4 * scale[2]
 
 
 def test_batched_64_dambreaks_bit_identical_and_smallest_against_oracle():

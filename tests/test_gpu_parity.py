@@ -329,12 +329,21 @@ def test_full_size_sort_and_lists_properties():
     nf = f.t["nbr_nf"][:n].cpu().numpy()
     assert nf.max() <= 64 and nf.min() >= 1
     assert 2 <= s.div_iter <= 101 and 2 <= s.incom_iter <= 101
-    # the block spans the tank's full z extent, so the state is mirror
-    # symmetric in z: the z walls' pushes cancel in the total z momentum
+    # The block spans the tank's full z extent, so the scene is mirror
+    # symmetric in z: the z walls' pushes cancel in the total z momentum,
+    # while the collapse drives the fluid along +x.  From rest the first two
+    # steps are pure gravity (v_x = v_z = 0), so the check runs after ten.
+    # Measured: |sum m v_z| / sum m |v_z| = 7.4e-3 after 10 steps (FP32
+    # rounding breaks the mirror at that level), against 0.25 along x.
+    for _ in range(9):
+        sim.sph_step()
     m = f.to_numpy("mass").astype(np.float64)
     v = f.to_numpy("vel").astype(np.float64)
     assert np.isfinite(v).all()
-    assert abs((m * v[:, 2]).sum()) <= 1e-4 * (m * np.abs(v[:, 2])).sum()
+    P, A = (m[:, None] * v).sum(0), (m[:, None] * np.abs(v)).sum(0)
+    print(f"  after 10 steps: momentum {P}, sum m|v| {A}", flush=True)
+    assert A[2] > 0 and abs(P[2]) <= 0.05 * A[2]
+    assert P[0] > 0.1 * A[0]
 
 
 def test_kept_boundary_sums_are_bit_identical():
