@@ -95,9 +95,10 @@ def test_reference_grid_bit_exact(name):
         assert np.array_equal(fl.to_numpy("neighb_in_cell_seq"), o.get(obj, "neighb_in_cell_seq"))
 
 
-def test_neighbor_search_lists_bit_exact():
+@pytest.mark.parametrize("name", ["dambreak_2d_8k", "dambreak_3d_8k"])
+def test_neighbor_search_lists_bit_exact(name):
     from tichi_sph_amd import NeighborSearch
-    sim, o = make("dambreak_2d_8k")
+    sim, o = make(name)
     ns = NeighborSearch(sim, 2)
     ns.establish_neighbs(sim.fluid, sim.bound)
     o.kernel("establish_neighbs", 0)
