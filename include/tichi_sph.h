@@ -55,6 +55,11 @@ enum { TSPH_DFSPH = 0, TSPH_VFSPH = 1 };
 #define TSPH_FLAG_FAST_QUOTIENT 1 /* FP32 fluid pass A may use the reduced sqrt and r/h
                                      sequences: set only after tsph_verify_fast_quotient
                                      found no difference for these parameters */
+#define TSPH_FLAG_ST_RECORDS 2    /* tsph_advect records +-psi (negative where the 0.99
+                                     gate fails) in the normal's spare lane also when
+                                     gamma = 0, for a surface-tension reverse that
+                                     takes d/d gamma at gamma = 0 (with gamma != 0 it
+                                     always does); otherwise that lane is 0 */
 
 typedef struct TsphParams {
   int32_t dim, phase_num, solver_type, precision;

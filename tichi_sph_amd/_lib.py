@@ -11,6 +11,7 @@ TSPH_MAX_PHASE = 8
 TSPH_MAX_ITER_LOG = 256
 ABI_VERSION = 4  # TSPH_ABI_VERSION of include/tichi_sph.h
 TSPH_FLAG_FAST_QUOTIENT = 1  # TsphParams.flags
+TSPH_FLAG_ST_RECORDS = 2
 TSPH_OBJ_STATIC_SUMS = 1  # TsphObject.flags
 TSPH_DFSPH = 0
 TSPH_VFSPH = 1

@@ -89,6 +89,21 @@ class DiffRunner:
         self.step_index = 0
         self.iter_log = []    # (div_iter, incom_iter) of every forward step since the last reset
 
+    @property
+    def surface_tension_grad(self):
+        return self._st_grad
+
+    @surface_tension_grad.setter
+    def surface_tension_grad(self, on):
+        """With the surface-tension reverse on, the forward advection records
+        what that reverse gathers also at gamma = 0 (TSPH_FLAG_ST_RECORDS)."""
+        self._st_grad = bool(on)
+        p = self.sim.params
+        if self._st_grad:
+            p.flags |= _lib.TSPH_FLAG_ST_RECORDS
+        else:
+            p.flags &= ~_lib.TSPH_FLAG_ST_RECORDS
+
     # -------------------------------------------------------------- params
     def set_params(self, mu=None, gamma=None):
         if mu is not None:

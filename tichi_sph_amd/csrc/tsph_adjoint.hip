@@ -286,18 +286,19 @@ __global__ void __launch_bounds__(kBlock)
       const T rhoi = dens[i], r0i = rdens[i], mi = fmass[i];
       int c[3];
       node_encode(k, p.x, p.y, p.z, c);
+      // per candidate: (x_j, X_j = m_j), the recorded normal with +-rho_j in
+      // its spare lane (negative where j's 0.99 gate fails; the forward
+      // advection writes it), vbar_j and rho0_j
       struct Cand {
         V4<T> q, nj, uj;
-        T vfj, rhoj, r0j, mj;
+        T r0j;
       };
-      auto load = [&](int j) {
-        return Cand{fpos[j], tnormal[j], vbar[j], vf[(long)j * k.P], dens[j], rdens[j], fmass[j]};
-      };
+      auto load = [&](int j) { return Cand{fpos[j], tnormal[j], vbar[j], rdens[j]}; };
       walk_cells_prefetch<kStU>(k, fstart, c, load, [&](int, const Cand& d) {
         const V4<T>& q = d.q;
         T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
         T r2 = dx * dx + dy * dy + dz * dz;
-        if (!(r2 > T(0)) || !(d.vfj > T(0.99))) return;
+        if (!(r2 > T(0)) || !(d.nj.w > T(0))) return;
         const T rinv = f_rsqrt(r2);
         T r = r2 * rinv;
         T C, Cp;
@@ -307,7 +308,7 @@ __global__ void __launch_bounds__(kBlock)
         const V4<T>& nj = d.nj;
         const V4<T>& uj = d.uj;
         T alj_x = dt * uj.x, alj_y = dt * uj.y, alj_z = dt * uj.z;
-        T rhoj = d.rhoj, r0j = d.r0j, mj = d.mj;
+        T rhoj = d.nj.w, r0j = d.r0j, mj = q.w;  // DFSPH: the alias X is the mass
         T den = rhoi + rhoj;
         const T rden = f_rcp(den);
         T fij = T(2) * r0i * rden, fji = T(2) * r0j * rden;

@@ -34,6 +34,7 @@ struct KP {
   int gs[3], coder[3];
   int list_cap_f, list_cap_b;
   int iter_min, iter_max, cs, wc_gamma;
+  int st_rec;    // advection writes the surface-tension records also at gamma = 0
   int fastq;     // FP32: q = r / h and sqrt by the verified reduced sequences (f_quot_h)
   T lb[3], g[3], rho[kMaxP];
   T h, h2, sig, sig_h, gps, c_coh, eps, r2max, inv_h;

@@ -87,6 +87,7 @@ KP<T> make_kp(const TsphParams& p) {
   k.wg_a = (T)(6.0 * (double)k.sig / ((double)h * (double)h));
   k.wg_b = (T)(6.0 * (double)k.sig / (double)h);
   k.fastq = (sizeof(T) == 4 && (p.flags & TSPH_FLAG_FAST_QUOTIENT)) ? 1 : 0;
+  k.st_rec = (p.flags & TSPH_FLAG_ST_RECORDS) ? 1 : 0;
   return k;
 }
 

@@ -598,7 +598,8 @@ __device__ __forceinline__ void k_advect_body(Blk blk, KP<T> k, int n, int cap, 
              const int* __restrict__ bstart, const V4<T>* __restrict__ bpos,
              const int* __restrict__ nbr_f, const int* __restrict__ nbr_nf,
              V4<T>* __restrict__ vadv, V4<T>* __restrict__ vel, V4<T>* __restrict__ acce_adv,
-             V4<T>* __restrict__ normal) {
+             V4<T>* __restrict__ normal, const T* __restrict__ fmass,
+             const T* __restrict__ psi_rho, V4<T>* __restrict__ strec) {
   const int base = xcd_block(blk.b, blk.nb) * kBlock;
   const int c = coop_lane<kCoop>();
 #pragma unroll 1
@@ -640,7 +641,18 @@ __device__ __forceinline__ void k_advect_body(Blk blk, KP<T> k, int n, int cap, 
     if (c != 0) continue;
     vel[i] = V4<T>{vi.x, vi.y, vi.z, T(0)};  // SPH_vel_adv_2_vel
     acce_adv[i] = V4<T>{k.g[0] + ax, k.g[1] + ay, k.g[2] + az, T(0)};
-    normal[i] = V4<T>{nx * k.h, ny * k.h, nz * k.h, T(0)};
+    if (k.gamma != T(0) || k.st_rec) {
+      // the records the surface-tension walks gather per candidate j: the
+      // normal's spare lane carries +-psi_j, negative where j's 0.99 gate
+      // fails (psi_j > 0 for every particle in a cell: its own W(0) term),
+      // and the forward walk's (x_j, m_j) goes to strec, so that it loads two
+      // records instead of five
+      const T psi = psi_rho[i];
+      normal[i] = V4<T>{nx * k.h, ny * k.h, nz * k.h, vf[(long)i * k.P] > T(0.99) ? psi : -psi};
+      strec[i] = V4<T>{p.x, p.y, p.z, fmass[i]};
+    } else {
+      normal[i] = V4<T>{nx * k.h, ny * k.h, nz * k.h, T(0)};
+    }
   }
 }
 
@@ -651,8 +663,9 @@ __global__ void __launch_bounds__(kBlock)
              const int* __restrict__ bstart, const V4<T>* __restrict__ bpos,
              const int* __restrict__ nbr_f, const int* __restrict__ nbr_nf,
              V4<T>* __restrict__ vadv, V4<T>* __restrict__ vel, V4<T>* __restrict__ acce_adv,
-             V4<T>* __restrict__ normal) {
-  k_advect_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, n, cap, fpos, vf, rdens, fstart, bstart, bpos, nbr_f, nbr_nf, vadv, vel, acce_adv, normal);
+             V4<T>* __restrict__ normal, const T* __restrict__ fmass,
+             const T* __restrict__ psi_rho, V4<T>* __restrict__ strec) {
+  k_advect_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, n, cap, fpos, vf, rdens, fstart, bstart, bpos, nbr_f, nbr_nf, vadv, vel, acce_adv, normal, fmass, psi_rho, strec);
 }
 
 // Surface tension loop 2 (sph.py:151-166) + SPH_advection_update_vel_adv
@@ -664,9 +677,10 @@ __global__ void __launch_bounds__(kBlock)
 constexpr int kStFwdU = 8;
 template <class T>
 __device__ __forceinline__ void k_surface_tension_body(Blk blk, KP<T> k, int n, const V4<T>* __restrict__ fpos,
-                      const T* __restrict__ fmass, const T* __restrict__ psi_rho,
+                      const T* __restrict__ psi_rho,
                       const T* __restrict__ rdens, const T* __restrict__ vf,
                       const int* __restrict__ fstart, const V4<T>* __restrict__ normal,
+                      const V4<T>* __restrict__ strec,
                       const V4<T>* __restrict__ vel, V4<T>* __restrict__ acce_adv,
                       V4<T>* __restrict__ vadv, const TsphControl* ctrl) {
   int i = xcd_block(blk.b, blk.nb) * kBlock + threadIdx.x;
@@ -680,25 +694,21 @@ __device__ __forceinline__ void k_surface_tension_body(Blk blk, KP<T> k, int n, 
     T psi_i = psi_rho[i];
     int c[3];
     node_encode(k, p.x, p.y, p.z, c);
-    struct Cand {
-      V4<T> q, nj;
-      T mj, psij, vfj;
-    };
-    auto load = [&](int j) {
-      return Cand{fpos[j], normal[j], fmass[j], psi_rho[j], vf[(long)j * k.P]};
-    };
-    walk_cells_prefetch<kStFwdU>(k, fstart, c, load, [&](int, const Cand& d) {
+    // per candidate: (x_j, m_j) and (n_j, +-psi_j), written by the advection
+    // sweep (psi_j < 0 where j's 0.99 gate fails)
+    auto load = [&](int j) { return Rec2<T>{strec[j], normal[j]}; };
+    walk_cells_prefetch<kStFwdU>(k, fstart, c, load, [&](int, const Rec2<T>& d) {
       // every candidate of the 27 cells: reciprocal square root and
       // reciprocal instead of the IEEE square root and divisions (the pair
       // terms of this walk are not part of the bit-exact sums)
-      const V4<T>& q = d.q;
+      const V4<T>& q = d.a;
       const T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
       const T r2 = dx * dx + dy * dy + dz * dz;
-      if (r2 > T(0) && d.vfj > T(0.99)) {
+      if (r2 > T(0) && d.b.w > T(0)) {
         const T rinv = f_rsqrt(r2);
-        const T cs = -k.gamma * d.mj * f_coh(k, r2 * rinv * k.inv_h);
-        const V4<T>& nj = d.nj;
-        const T f = T(2) * rp * f_rcp(psi_i + d.psij);
+        const T cs = -k.gamma * q.w * f_coh(k, r2 * rinv * k.inv_h);
+        const V4<T>& nj = d.b;
+        const T f = T(2) * rp * f_rcp(psi_i + nj.w);
         const T csr = cs * rinv;
         a.x += f * (csr * dx + k.gamma * (ni.x - nj.x));
         a.y += f * (csr * dy + k.gamma * (ni.y - nj.y));
@@ -732,12 +742,13 @@ __global__ void __launch_bounds__(kBlock)
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_surface_tension(KP<T> k, int n, const V4<T>* __restrict__ fpos,
-                      const T* __restrict__ fmass, const T* __restrict__ psi_rho,
+                      const T* __restrict__ psi_rho,
                       const T* __restrict__ rdens, const T* __restrict__ vf,
                       const int* __restrict__ fstart, const V4<T>* __restrict__ normal,
+                      const V4<T>* __restrict__ strec,
                       const V4<T>* __restrict__ vel, V4<T>* __restrict__ acce_adv,
                       V4<T>* __restrict__ vadv, const TsphControl* ctrl) {
-  k_surface_tension_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, n, fpos, fmass, psi_rho, rdens, vf, fstart, normal, vel, acce_adv, vadv, ctrl);
+  k_surface_tension_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, n, fpos, psi_rho, rdens, vf, fstart, normal, strec, vel, acce_adv, vadv, ctrl);
 }
 
 // ---------------------------------------------------------------------------
@@ -1175,20 +1186,21 @@ int advect(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl* ctr
                "tsph_advect: workspace too small");
   if (f->n == 0) return 0;
   int nb = blocks_for(f->n);
+  const T* psi = (const T*)(prm.solver_type == TSPH_VFSPH ? f->sph_compression : f->sph_density);
   hipLaunchKernelGGL(k_advect<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
                      (const V4<T>*)f->pos, (const T*)f->volume_frac, (const T*)f->rest_density,
                      f->cell_start, b->cell_start, (const V4<T>*)b->pos, f->nbr_f, f->nbr_nf,
                      (V4<T>*)f->vel_adv, (V4<T>*)f->vel, (V4<T>*)f->acce_adv,
-                     (V4<T>*)f->normal);
-  const T* psi = (const T*)(prm.solver_type == TSPH_VFSPH ? f->sph_compression : f->sph_density);
+                     (V4<T>*)f->normal, (const T*)f->mass, psi, (V4<T>*)L.krec);
   if (k.gamma == T(0))
     hipLaunchKernelGGL(k_vel_adv_update<T>, dim3(nb), dim3(kBlock), 0, st, f->n,
                        (const V4<T>*)f->vel, (const V4<T>*)f->acce_adv, (V4<T>*)f->vel_adv, ctrl);
   else
     hipLaunchKernelGGL(k_surface_tension<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n,
-                       (const V4<T>*)f->pos, (const T*)f->mass, psi, (const T*)f->rest_density,
+                       (const V4<T>*)f->pos, psi, (const T*)f->rest_density,
                        (const T*)f->volume_frac, f->cell_start, (const V4<T>*)f->normal,
-                       (const V4<T>*)f->vel, (V4<T>*)f->acce_adv, (V4<T>*)f->vel_adv, ctrl);
+                       (const V4<T>*)L.krec, (const V4<T>*)f->vel, (V4<T>*)f->acce_adv,
+                       (V4<T>*)f->vel_adv, ctrl);
   TSPH_LAUNCH_CHECK("advect");
   return 0;
 }
@@ -1489,7 +1501,8 @@ __global__ void __launch_bounds__(kBlock) kb_advect(const char* __restrict__ tab
     k_advect_body<T>(blk, e.k, f.n, f.capacity, (const V4<T>*)f.pos, (const T*)f.volume_frac,
                      (const T*)f.rest_density, f.cell_start, e.b.cell_start,
                      (const V4<T>*)e.b.pos, f.nbr_f, f.nbr_nf, (V4<T>*)f.vel_adv, (V4<T>*)f.vel,
-                     (V4<T>*)f.acce_adv, (V4<T>*)f.normal);
+                     (V4<T>*)f.acce_adv, (V4<T>*)f.normal, (const T*)f.mass, psi_of(e),
+                     (V4<T>*)e.L.krec);
   });
 }
 
@@ -1497,10 +1510,10 @@ template <class T>
 __global__ void __launch_bounds__(kBlock) kb_surface_tension(const char* __restrict__ tab, int ns) {
   for_tiles<T, kTileFluid>(tab, ns, [&](const SceneEntry<T>& e, int, Blk blk) {
     const TsphObject& f = e.f;
-    k_surface_tension_body<T>(blk, e.k, f.n, (const V4<T>*)f.pos, (const T*)f.mass, psi_of(e),
+    k_surface_tension_body<T>(blk, e.k, f.n, (const V4<T>*)f.pos, psi_of(e),
                               (const T*)f.rest_density, (const T*)f.volume_frac, f.cell_start,
-                              (const V4<T>*)f.normal, (const V4<T>*)f.vel, (V4<T>*)f.acce_adv,
-                              (V4<T>*)f.vel_adv, e.ctrl);
+                              (const V4<T>*)f.normal, (const V4<T>*)e.L.krec, (const V4<T>*)f.vel,
+                              (V4<T>*)f.acce_adv, (V4<T>*)f.vel_adv, e.ctrl);
   });
 }
 
