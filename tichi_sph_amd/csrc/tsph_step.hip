@@ -759,7 +759,7 @@ __global__ void __launch_bounds__(kBlock)
     k_fbm_drift(KP<T> k, int n, const V4<T>* __restrict__ vadv, const V4<T>* __restrict__ vel,
                 const T* __restrict__ rdens, const T* __restrict__ vf, V4<T>* __restrict__ acce_adv,
                 V4<T>* __restrict__ acce, T* __restrict__ zeta, V4<T>* __restrict__ drift,
-                T* __restrict__ tmp, TsphControl* ctrl) {
+                T* __restrict__ tmp, const int* __restrict__ flag, TsphControl* ctrl) {
   TSPH_STRICT
   if (!ctrl->general_flag) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->fbm_rounds += 1;
@@ -791,7 +791,10 @@ __global__ void __launch_bounds__(kBlock)
     }
     for (int d = 0; d < 3; ++d) u[d] = (u[d] + (fa[d] - aa[d])) * dt;
     if (k.dim == 2) u[2] = T(0);
-    drift[(long)i * P + q] = V4<T>{u[0], u[1], u[2], T(0)};
+    // the spare lane carries what the cooperative flux sweep gathers of this
+    // particle besides its drift: phi_q, or -1 once the particle is flagged
+    // (sph.py:366-372; the flags of this round are those of the last check)
+    drift[(long)i * P + q] = V4<T>{u[0], u[1], u[2], flag[i] != 0 ? T(-1) : phi};
   }
 }
 
@@ -863,10 +866,8 @@ __global__ void __launch_bounds__(kBlock)
 template <class T, int NP>
 struct FbmRec {
   V4<T> x;
-  V4<T> u[NP];
-  T phi[NP];
+  V4<T> u[NP];  // drift of each phase, (u, phi) or (u, -1) when flagged (k_fbm_drift)
   T V;
-  int fl;
 };
 
 template <class T, int NP>
@@ -896,20 +897,18 @@ __global__ void __launch_bounds__(kBlock)
       conv[s] = T(0);
       diff[s] = T(0);
     }
+    // VFSPH: the position's alias lane is the rest volume (no extra gather)
+    const bool vx = k.solver == TSPH_VFSPH;
     auto load = [&](int j) {
       FbmRec<T, NP> d;
       d.x = fpos[j];
 #pragma unroll
-      for (int s = 0; s < NP; ++s) {
-        d.u[s] = drift[(long)j * NP + s];
-        d.phi[s] = vf[(long)j * NP + s];
-      }
-      d.V = rvol[j];
-      d.fl = flag[j];
+      for (int s = 0; s < NP; ++s) d.u[s] = drift[(long)j * NP + s];
+      d.V = vx ? d.x.w : rvol[j];
       return d;
     };
     auto pair = [&](const FbmRec<T, NP>& d) {
-      if (d.fl != 0) return;
+      if (d.u[0].w < T(0)) return;  // a flagged neighbour
       const T dx = p.x - d.x.x, dy = p.y - d.x.y, dz = p.z - d.x.z;
       const T r2 = dx * dx + dy * dy + dz * dz;
       T r;
@@ -918,11 +917,12 @@ __global__ void __launch_bounds__(kBlock)
       const T gd = gv * r2 * f_rcp(r2 + k.eps);
 #pragma unroll
       for (int s = 0; s < NP; ++s) {
-        const T fx = pi[s] * ui[s].x + d.phi[s] * d.u[s].x;
-        const T fy = pi[s] * ui[s].y + d.phi[s] * d.u[s].y;
-        const T fz = pi[s] * ui[s].z + d.phi[s] * d.u[s].z;
+        const T pj = d.u[s].w;
+        const T fx = pi[s] * ui[s].x + pj * d.u[s].x;
+        const T fy = pi[s] * ui[s].y + pj * d.u[s].y;
+        const T fz = pi[s] * ui[s].z + pj * d.u[s].z;
         conv[s] += (fx * dx + fy * dy + fz * dz) * gv;
-        diff[s] += (pi[s] - d.phi[s]) * gd;
+        diff[s] += (pi[s] - pj) * gd;
       }
     };
     sweep_lists_coop<kCoop, T, FbmRec<T, NP>>(
@@ -1217,7 +1217,7 @@ int fbm_round(const TsphParams& prm, TsphObject* f, TsphControl* ctrl, void* ws,
   hipLaunchKernelGGL(k_fbm_drift<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n,
                      (const V4<T>*)f->vel_adv, (const V4<T>*)f->vel, (const T*)f->rest_density,
                      (const T*)f->volume_frac, (V4<T>*)f->acce_adv, (V4<T>*)f->acce,
-                     (T*)f->fbm_zeta, (V4<T>*)f->drift_vel, (T*)f->volume_frac_tmp, ctrl);
+                     (T*)f->fbm_zeta, (V4<T>*)f->drift_vel, (T*)f->volume_frac_tmp, f->flag, ctrl);
   auto flux = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
                        (const V4<T>*)f->pos, (const T*)f->rest_volume, (const T*)f->volume_frac,
