@@ -54,6 +54,29 @@ def test_batched_step_bit_identical_to_single_scenes():
     print(f"  iterations per scene: {list(zip(di.tolist(), ii.tolist()))}", flush=True)
 
 
+def test_batch_follows_a_scene_stepped_on_its_own():
+    """A scene of a batch stepped by its own sph_step() between batched steps
+    (its fluid is sorted on the host's path: front and back buffers trade
+    places): the next batched step rebuilds the scene table and every scene
+    stays bit-identical to the same sequence run scene by scene."""
+    from tichi_sph_amd.batch import SceneBatch
+    d = scene_dicts()
+    print(f"[case] batch of {len(d)} scenes with a single-scene step in between", flush=True)
+    ref = make_sims(d)
+    batch = SceneBatch(make_sims(d))
+    batch.step()
+    for sim in ref:
+        sim.sph_step()
+    for s in (0, 3):
+        batch.sims[s].sph_step()
+        ref[s].sph_step()
+    for step in range(2):
+        batch.step()
+        for sim in ref:
+            sim.sph_step()
+        assert_same(batch.sims, ref, f"after the single-scene steps, batched step {step}")
+
+
 def test_batched_fp64_step_bit_identical_to_single_scenes():
     """The grouped kernels' FP64 instantiation (the precision of the
     differentiable runs) against the FP64 single-scene steps."""

@@ -58,7 +58,6 @@ class SceneBatch:
         self.cfl = 1 if cfl else 0
         ns = len(sims)
         self.ns = ns
-        self._fl = (_lib.TsphObject * ns)(*[s.fluid.struct() for s in sims])
         self._ctrl = (ctypes.c_void_p * ns)(*[s.ctrl.data_ptr() for s in sims])
         self._ws = (ctypes.c_void_p * ns)(*[s.ws.data_ptr() for s in sims])
         self._wsb = (ctypes.c_size_t * ns)(*[s.ws.numel() for s in sims])
@@ -89,6 +88,8 @@ class SceneBatch:
     def _prepare(self):
         """(Re)build the device scene table from the scenes' descriptors."""
         ns = self.ns
+        self._fl = (_lib.TsphObject * ns)(*[s.fluid.struct() for s in self.sims])
+        self._layouts = [s.fluid.layout_epoch for s in self.sims]
         self._params = (_lib.TsphParams * ns)(*[s.params for s in self.sims])
         self._bo = (_lib.TsphObject * ns)(*[s.bound.struct() for s in self.sims])
         self._epochs = [s.bound.sums_epoch for s in self.sims]
@@ -98,21 +99,40 @@ class SceneBatch:
             self.table.numel(), ctypes.byref(self.info), self._stream()), "tsph_batch_prepare")
         torch.cuda.current_stream(self.device).synchronize()  # staging copied
 
-    def _refresh_bounds(self):
-        """A scene whose boundary changed since the table was built (new or
-        moved boundary particles) is re-sorted and the table rebuilt, so no
-        batched step searches a stale boundary cell order.  The table is also
-        rebuilt when a boundary's kept (b,b) sums were invalidated (written
-        positions or masses): the grouped pass A marks the table's copy of
-        the flag, which only a rebuild from the host descriptors clears."""
-        stale = [s for s in self.sims if s.bound_needs_sort()]
-        epochs = any(s.bound.sums_epoch != e for s, e in zip(self.sims, self._epochs))
-        if not stale and not epochs:
+    def _refresh(self):
+        """Rebuild the device scene table when a scene changed on the host
+        since it was built:
+        * a boundary with new or moved particles is re-sorted first, so no
+          batched step searches a stale boundary cell order;
+        * a boundary whose kept (b,b) sums were invalidated (written
+          positions or masses): the grouped pass A marks the table's copy of
+          the flag, which only a rebuild from the host descriptors clears;
+        * a fluid sorted on its own (the scene's sph_step) or grown
+          (push_part_seq): its front/back buffers or count differ from the
+          table's.
+        Other host-side changes (a scene's parameters) need refresh()."""
+        stale, changed = [], False
+        for s, e, le in zip(self.sims, self._epochs, self._layouts):
+            if s.bound_needs_sort():
+                stale.append(s)
+            if s.bound.sums_epoch != e or s.fluid.layout_epoch != le:
+                changed = True
+        if not stale and not changed:
             return
         if self.graphs is not None:
-            raise RuntimeError("a scene's boundary changed after capture(): release_graphs() first")
+            raise RuntimeError("a scene changed after capture(): release_graphs() first")
         for s in stale:
             s._sort_bound()
+        self._prepare()
+
+    def refresh(self):
+        """Rebuild the scene table from the scenes as they are now (after a
+        change of a scene's parameters, for instance)."""
+        if self.graphs is not None:
+            raise RuntimeError("release_graphs() before refresh()")
+        for s in self.sims:
+            if s.bound_needs_sort():
+                s._sort_bound()
         self._prepare()
 
     def __del__(self):
@@ -234,7 +254,7 @@ class SceneBatch:
     # ----------------------------------------------------------------- step
     def step(self):
         """One sph_step of every scene; returns ([div_iter], [incom_iter])."""
-        self._refresh_bounds()
+        self._refresh()
         if self.graphs is not None:
             return self._step_graph()
         self._sort()

@@ -88,6 +88,7 @@ class Fluid:
         self._t = {}
         self._pending_swaps = 0  # front/back swaps done on the device, not yet mirrored here
         self.sums_epoch = 0  # invalidations of the kept boundary sums (invalidate_static_sums)
+        self.layout_epoch = 0  # host-side storage changes: single-object sorts, added particles
         for name in VEC_FIELDS:
             self.t[name] = z(cap, 4)
         for name in SCALAR_FIELDS:
@@ -162,6 +163,7 @@ class Fluid:
         inside the descriptor, swap the tensors so that self.t matches the
         descriptor again."""
         self.cell_table_ok = True
+        self.layout_epoch += 1  # a SceneBatch holding this object's descriptor rebuilds it
         s = self._struct
         for name in ("pos", "vel", "mass", "rest_density", "rest_volume", "volume_frac", "pid"):
             if getattr(s, name) != self.t[name].data_ptr():
@@ -325,6 +327,7 @@ class Fluid:
         self.t["pid"][sl] = torch.arange(n0, n1, dtype=torch.int32, device=self.device)
         self.color += [color] * int(pushed_part_num)
         self.part_num = n1
+        self.layout_epoch += 1
         self.cell_table_ok = False  # the next step sorts the object again
         self.invalidate_static_sums()
         self.update_mass_all()
