@@ -2,7 +2,9 @@
 a grouped launch over a scene table runs the same per-block bodies as the
 single-scene kernels, so every scene of a batch must match its own
 Simulation.sph_step() bit for bit, in eager mode, under hipGraph replay and
-for every grouped-launch block count."""
+for every grouped-launch block count.  The batch mixes 2D and 3D DFSPH dam
+breaks with a 3D VFSPH two-phase mixture (surface tension, per-phase
+viscosity)."""
 import numpy as np
 import pytest
 import torch
@@ -15,7 +17,9 @@ pytestmark = pytest.mark.gpu
 def scene_dicts():
     from tichi_sph_amd.scenarios.make_scenarios import batched_dambreaks
     out = [(n, s, c) for n, s, c in batched_dambreaks(n_scenes=4, seed=1, n_min=2e3, n_max=2e4)]
-    for name in ("dambreak_3d_8k", "dambreak_2d_8k"):
+    # a VFSPH two-phase mixture with surface tension and per-phase viscosity
+    # (no FBM: not batched) beside the DFSPH dam breaks
+    for name in ("dambreak_3d_8k", "two_phase_3d_4k", "dambreak_2d_8k"):
         cb, sb = load_scene(name)
         out.append((name, sb, cb))
     return out
