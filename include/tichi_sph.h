@@ -18,6 +18,11 @@
  *   pos.w holds the solver alias X (mass for DFSPH, rest_volume for VFSPH,
  *   sph_obj.py:65-73) so one 16-byte load fetches position and X.
  *   volume_frac is [capacity][phase_num]; drift_vel is [capacity][phase_num][4].
+ *   The other w lanes are not part of the reference's vectors: the fused step
+ *   keeps records there for later sweeps (vel_adv.w = X / psi between pass A
+ *   and the advection; normal.w = +-psi, negative where the 0.99
+ *   surface-tension gate fails, when gamma != 0 or TSPH_FLAG_ST_RECORDS is
+ *   set; drift_vel.w = phi of that phase, or -1 once flagged, in an FBM round).
  *   The fused step keeps particles in cell order (a counting sort each step);
  *   `pid` holds each slot's reference particle index (creation order).
  */
