@@ -108,6 +108,36 @@ def test_dambreak_1m_moving_step_matches_oracle():
         assert e32 <= TOL or eg <= 2 * eo + TOL / 10, name
 
 
+def test_dambreak_1m_300_steps_stay_physical():
+    """300 steps of the 1M dam break (30 ms of flow): no list overflow, every
+    solve within its iteration bounds, a finite state, constant total mass,
+    the centre of mass moving down and along +x as the column collapses, and
+    the potential energy falling.  The oracle would take about ten minutes
+    for this run, so these size-independent properties stand in for it."""
+    from tichi_sph_amd import load_simulation
+    print("[case] dambreak_3d_1m, 300 steps", flush=True)
+    sim = load_simulation(os.path.join(SCEN, "dambreak_3d_1m.config.json"),
+                          os.path.join(SCEN, "dambreak_3d_1m.json"))
+    f = sim.fluid
+    m0 = f.to_numpy("mass").astype(np.float64)
+    x0 = (m0[:, None] * f.to_numpy("pos").astype(np.float64)).sum(0) / m0.sum()
+    pe = []
+    for step in range(1, 301):
+        st = sim.sph_step(stats=True)
+        assert st.overflow_f == 0 and st.overflow_b == 0, step
+        assert 2 <= st.div_iter <= 101 and 2 <= st.incom_iter <= 101, step
+        if step % 50 == 0:
+            pe.append(st.gravity_potential_energy)
+            assert np.isfinite(f.to_numpy("vel")).all(), step
+    m = f.to_numpy("mass").astype(np.float64)
+    x = (m[:, None] * f.to_numpy("pos").astype(np.float64)).sum(0) / m.sum()
+    print(f"  centre of mass {x0} -> {x}; potential energy every 50 steps {pe}; "
+          f"last iterations ({st.div_iter}, {st.incom_iter})", flush=True)
+    assert np.array_equal(m, m0)
+    assert x[1] < x0[1] and x[0] > x0[0]
+    assert all(b < a for a, b in zip(pe, pe[1:]))
+
+
 def test_batched_64_dambreaks_bit_identical_and_smallest_against_oracle():
     from tichi_sph_amd import simulation_from_dicts
     from tichi_sph_amd.batch import SceneBatch, iteration_counts
