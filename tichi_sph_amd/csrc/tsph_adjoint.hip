@@ -156,6 +156,11 @@ __global__ void __launch_bounds__(kBlock)
   const T rmi = T(1) / Xi;
   const T wx = u.x * rmi, wy = u.y * rmi, wz = u.z * rmi;
   const T ki = u.w;
+  // the particle's own fields, requested before the walk so that their
+  // latency hides behind it (only this particle's group writes them)
+  const T al = alpha[i];
+  const T rb0 = rhobar[i], ab0 = alphabar[i];
+  const V4<T> xb = xbar[i];
   T kb = 0, ax = 0, ay = 0, az = 0;
   auto term = [&](const Rec2<T>& d, bool fluid) {
     const V4<T>& q = d.a;   // q.w = X_j = m_j
@@ -180,12 +185,10 @@ __global__ void __launch_bounds__(kBlock)
   if (cl != 0) continue;
   const T s = -(T(1) / dt);
   kb *= s;
-  const T al = alpha[i];
   const T pb = ki > T(0) ? kb / al : T(0);
   tvel[i].w = pb;
-  rhobar[i] += pb;
-  alphabar[i] += -kb * ki / al;
-  V4<T> xb = xbar[i];
+  rhobar[i] = rb0 + pb;
+  alphabar[i] = ab0 + -kb * ki / al;
   xbar[i] = v4(xb.x + s * ax, xb.y + s * ay, xb.z + s * az);
   }
 }
@@ -217,6 +220,11 @@ __global__ void __launch_bounds__(kBlock)
   const T Xi = p.w;
   const V4<T> vi = tvel[i];  // w = psi_bar of this particle
   const T pbi = vi.w;
+  // the particle's own fields, requested before the walk (only this
+  // particle's group writes xbar[i])
+  const V4<T> u = vbar[i];
+  const T kn = kappa_next ? kappa_next[i] : T(0);
+  const V4<T> xb = xbar[i];
   T vx = 0, vy = 0, vz = 0, ax = 0, ay = 0, az = 0;
   auto term = [&](const Rec2<T>& d) {
     const V4<T>& q = d.a;   // q.w = X_j
@@ -241,10 +249,7 @@ __global__ void __launch_bounds__(kBlock)
   ay = group_sum<kCoop>(ay);
   az = group_sum<kCoop>(az);
   if (cl != 0) continue;
-  const V4<T> u = vbar[i];
-  vbar_out[i] = v4(u.x + dt * vx, u.y + dt * vy, u.z + dt * vz,
-                   kappa_next ? kappa_next[i] : T(0));
-  V4<T> xb = xbar[i];
+  vbar_out[i] = v4(u.x + dt * vx, u.y + dt * vy, u.z + dt * vz, kn);
   xbar[i] = v4(xb.x + dt * ax, xb.y + dt * ay, xb.z + dt * az);
   }
 }

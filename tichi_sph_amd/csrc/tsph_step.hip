@@ -429,6 +429,12 @@ __device__ __forceinline__ void k_solver_psi_body(Blk blk, KP<T> k, int n, int c
     if (i >= n) break;
     const V4<T> p = fpos[i];
     const V4<T> va = vadv[i];
+    // the particle's own fields, requested before the walk so that their
+    // latency hides behind it
+    const V4<T> B = a1b[i];
+    const T rp = rest_psi_of(k, rdens, i);
+    const T pr = psi_rho[i];
+    const T al = alpha[i];
     T sum = 0;  // this lane's share of sum_j X_j (v_i - v_j) . grad W_ij
     auto term = [&](const Rec2<T>& d) {
       const V4<T>& q = d.a;
@@ -448,16 +454,14 @@ __device__ __forceinline__ void k_solver_psi_body(Blk blk, KP<T> k, int n, int c
         [&](int) { return Rec2<T>{}; }, term, [&](const Rec2<T>&) {});
     sum = group_sum<kCoop>(sum);
     if (c != 0) continue;
-    const V4<T> B = a1b[i];
     sum += va.x * B.x + va.y * B.y + va.z * B.z;
-    const T rp = rest_psi_of(k, rdens, i);
-    T psi = (psi_rho[i] - rp) + sum * dt;
+    T psi = (pr - rp) + sum * dt;
     // IPPE_psi_adv_non_negative (sph.py:219-226); with a clamp reference, the
     // clamp set of a recorded iteration instead (psi clamped exactly where the
     // recorded kappa is 0): the path whose derivative the adjoint takes
     if (clamp_ref ? clamp_ref[i] == T(0) : psi < T(0)) psi = T(0);
     psi_adv[i] = psi;
-    const T kap = psi / alpha[i];
+    const T kap = psi / al;
     kappa[i] = kap;
     krec[i] = V4<T>{p.x, p.y, p.z, kap};  // the record the update sweep gathers
     if (tape_v) {  // record the iteration for the adjoint
