@@ -20,7 +20,8 @@
  *   volume_frac is [capacity][phase_num]; drift_vel is [capacity][phase_num][4].
  *   The other w lanes are not part of the reference's vectors: the fused step
  *   keeps records there for later sweeps (vel_adv.w = X / psi between pass A
- *   and the advection; normal.w = +-psi, negative where the 0.99
+ *   and the advection, and vel.w the same between the advection and the
+ *   tail, which writes 0; normal.w = +-psi, negative where the 0.99
  *   surface-tension gate fails, when gamma != 0 or TSPH_FLAG_ST_RECORDS is
  *   set; drift_vel.w = phi of that phase, or -1 once flagged, in an FBM round).
  *   The fused step keeps particles in cell order (a counting sort each step);
@@ -364,7 +365,7 @@ typedef struct TsphAdjoint {
   void *kappa_next;   /* T: recorded kappa (psi_adv / alpha) of the iteration that the next
                          tsph_adj_solver_iteration call reverses, or NULL if none: every
                          call that writes the velocity adjoint stores it in vbar's w lane */
-  void *tape_vel;     /* V4: recorded velocity after the divergence solve */
+  void *tape_vel;     /* V4: recorded velocity after the advection (its w lane: X / psi) */
   void *tape_normal;  /* V4: recorded surface-tension normals */
   double *grad;       /* [2]: accumulated d loss / d dynamic_viscosity, d surface_tension_gamma */
 } TsphAdjoint;

@@ -366,7 +366,7 @@ struct Rec4 {
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_adj_viscosity(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
-                    const T* __restrict__ vol, const T* __restrict__ dens,
+                    const T* __restrict__ dens,
                     const T* __restrict__ rdens, const int* __restrict__ fstart,
                     const int* __restrict__ bstart, const V4<T>* __restrict__ bpos,
                     const int* __restrict__ nbr_f, const int* __restrict__ nbr_nf,
@@ -389,9 +389,14 @@ __global__ void __launch_bounds__(kBlock)
     const V4<T> ui = vbar[i];
     const V4<T> nbi = nbar ? nbar[i] : V4<T>{};
     const T ax_ = dt * ui.x, ay_ = dt * ui.y, az_ = dt * ui.z;
-    const T Vi = vol[i];
+    const T Vi = vi.w;  // the recorded velocity's spare lane holds V_i = X_i / psi_i
     const T fmu_i = k.two_d2 / rdens[i];  // fv_i / mu
     const T fvi = fmu_i * k.mu;
+    // the particle's own outputs' inputs, requested before the walk (only
+    // this particle's group writes rhobar[i] and xbar[i])
+    const T kn = kappa_next ? kappa_next[i] : T(0);
+    const T rb0 = rhobar[i], di = dens[i];
+    const V4<T> xb = xbar[i];
     T vbx = 0, vby = 0, vbz = 0, xbx = 0, xby = 0, xbz = 0, Vb = 0, ms = 0;
     auto term = [&](const Rec4<T>& d) {
       const V4<T>& q = d.a;   // position
@@ -432,8 +437,7 @@ __global__ void __launch_bounds__(kBlock)
     sweep_lists_coop<kCoop, T, Rec4<T>>(
         k, cap, i, cl, p, fstart, bstart, fpos, bpos, nbr_f, nullptr, nbr_nf[i], 0,
         [&](int j) {
-          V4<T> v = tvel[j];
-          v.w = vol[j];
+          const V4<T> v = tvel[j];  // w = V_j
           V4<T> u = vbar[j];
           V4<T> a = v4(dt * u.x, dt * u.y, dt * u.z, k.two_d2 * k.mu / rdens[j]);
           return Rec4<T>{fpos[j], v, a, nbar ? nbar[j] : V4<T>{}};
@@ -448,9 +452,8 @@ __global__ void __launch_bounds__(kBlock)
     Vb = group_sum<kCoop>(Vb);
     ms = group_sum<kCoop>(ms);
     if (cl != 0) continue;
-    vbar_out[i] = v4(ui.x + vbx, ui.y + vby, ui.z + vbz, kappa_next ? kappa_next[i] : T(0));
-    rhobar[i] += -Vb * Vi / dens[i];
-    V4<T> xb = xbar[i];
+    vbar_out[i] = v4(ui.x + vbx, ui.y + vby, ui.z + vbz, kn);
+    rhobar[i] = rb0 + -Vb * Vi / di;
     xbar[i] = v4(xb.x + xbx, xb.y + xby, xb.z + xbz);
     msum += (double)ms;
   }
@@ -509,6 +512,7 @@ __global__ void __launch_bounds__(kBlock)
   const T rbi = rhobar[i];
   const V4<T> abi = a1bar[i];
   const T xm_i = Xi * Xi / mi;
+  const V4<T> xb = xbar[i];  // requested before the walk; only this group writes it
   T ax = 0, ay = 0, az = 0;
   // record: a = position (w = X_j), b = a1bar_j (w = a2bar_j),
   // c = (rho_bar_j, m_j, is_fluid)
@@ -554,7 +558,6 @@ __global__ void __launch_bounds__(kBlock)
   ay = group_sum<kCoop>(ay);
   az = group_sum<kCoop>(az);
   if (cl != 0) continue;
-  V4<T> xb = xbar[i];
   xbar[i] = v4(xb.x + ax, xb.y + ay, xb.z + az);
   }
 }
@@ -627,7 +630,7 @@ int adj_advect(const TsphParams& prm, TsphObject* f, TsphControl* ctrl, TsphAdjo
     hipLaunchKernelGGL(k_adj_grad_add, dim3(1), dim3(kBlock), 0, st, L.partial, nb, a->grad, 1);
   }
   hipLaunchKernelGGL(k_adj_viscosity<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
-                     (const V4<T>*)f->pos, (const T*)f->vol, (const T*)f->sph_density,
+                     (const V4<T>*)f->pos, (const T*)f->sph_density,
                      (const T*)f->rest_density, f->cell_start, (const int*)nullptr,
                      (const V4<T>*)nullptr, f->nbr_f, f->nbr_nf, (const V4<T>*)a->tape_vel,
                      (const V4<T>*)a->nbar, (const V4<T>*)a->vbar, (const T*)a->kappa_next,

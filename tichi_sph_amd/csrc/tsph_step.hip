@@ -643,7 +643,9 @@ __device__ __forceinline__ void k_advect_body(Blk blk, KP<T> k, int n, int cap, 
     ny = group_sum<kCoop>(ny);
     nz = group_sum<kCoop>(nz);
     if (c != 0) continue;
-    vel[i] = V4<T>{vi.x, vi.y, vi.z, T(0)};  // SPH_vel_adv_2_vel
+    // SPH_vel_adv_2_vel; the spare lane keeps V_i until the tail writes 0:
+    // the adjoint's tape of this velocity carries it to the reverse sweep
+    vel[i] = vi;
     acce_adv[i] = V4<T>{k.g[0] + ax, k.g[1] + ay, k.g[2] + az, T(0)};
     if (k.gamma != T(0) || k.st_rec) {
       // the records the surface-tension walks gather per candidate j: the
