@@ -81,7 +81,7 @@ def copy_calibration(mib, reps=5):
                       "kernel": "vectorized_elementwise_kernel (MulFunctor)"}), flush=True)
 
 
-TAPE_BUDGET = 8 << 30  # bytes of HBM for kept segment tapes in the differentiable legs
+TAPE_BUDGET = 32 << 30  # bytes of HBM (of 288 GB) for kept segment tapes in the differentiable legs
 
 
 def adjoint_leg(scen, steps, every):

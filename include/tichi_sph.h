@@ -394,6 +394,12 @@ int tsph_solver_psi_frozen(const TsphParams* p, TsphObject* fluid, TsphObject* b
 int tsph_relist(const TsphParams* p, TsphObject* fluid, TsphObject* bound, TsphControl* ctrl,
                 void* workspace, size_t ws_bytes, void* stream);
 
+/* tsph_relist for the boundary lists (nbr_b, nbr_nb) alone, for a caller
+ * that stored the step's fluid lists (nbr_f, nbr_nf) and points the fluid
+ * descriptor at that copy. */
+int tsph_relist_bound(const TsphParams* p, TsphObject* fluid, TsphObject* bound,
+                      TsphControl* ctrl, void* workspace, size_t ws_bytes, void* stream);
+
 /* Opens the reverse of one step (tail, SPH_update_pos sph.py:265-269):
  * reads d loss / d pos and d loss / d vel after the step from V4 arrays in
  * reference (pid) order, gathers them into the step's cell order with

@@ -212,13 +212,24 @@ def test_position_adjoint_matches_fd_fp64():
 def test_checkpoint_interval_does_not_change_gradient():
     """Replays are deterministic: every = 1 and every = steps, and tape
     budgets that keep none, some or all of the earlier segments' tapes, give
-    bit-identical gradients."""
+    bit-identical gradients; so do tapes with the step's fluid lists stored
+    (the reverse rebuilds only the boundary lists) and without (it rebuilds
+    all of them)."""
     name, steps = "dambreak_3d_8k", 6
     r, st0, v0, target, rollout = setup_case(name, steps, precision=4)
     r.restore(st0)
-    one = r.step(tape=True).nbytes()
+    r.tape_lists = False
+    bare = r.step(tape=True)
+    r.restore(st0)
+    r.tape_lists = True
+    full = r.step(tape=True)
+    assert bare.nbr_f is None and full.nbr_f is not None
+    assert full.nbytes() > bare.nbytes()
+    one = full.nbytes()
     out = []
-    for every, budget in ((1, 0), (steps, 0), (2, 0), (2, int(2.5 * one)), (2, 1 << 40), (1, 1 << 40)):
+    for tl, every, budget in [(tl, e, b) for tl in (True, False) for e, b in
+                              ((1, 0), (steps, 0), (2, 0), (2, int(2.5 * one)), (2, 1 << 40), (1, 1 << 40))]:
+        r.tape_lists = tl
         r.restore(st0)
         mu = torch.tensor(0.01, dtype=torch.float32, device=v0.device, requires_grad=True)
         gam = torch.tensor(0.07, dtype=torch.float32, device=v0.device, requires_grad=True)

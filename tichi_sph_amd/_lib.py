@@ -108,7 +108,7 @@ EXPORTED = [
     "tsph_verify_fast_quotient",
     "tsph_solver_iteration", "tsph_solver_psi", "tsph_solver_update", "tsph_advect",
     "tsph_fbm_round", "tsph_tail",
-    "tsph_ref_kernel", "tsph_solver_psi_taped", "tsph_solver_psi_frozen", "tsph_relist", "tsph_adj_begin",
+    "tsph_ref_kernel", "tsph_solver_psi_taped", "tsph_solver_psi_frozen", "tsph_relist", "tsph_relist_bound", "tsph_adj_begin",
     "tsph_adj_solver_iteration", "tsph_adj_advect", "tsph_adj_density_alpha", "tsph_adj_end",
     "tsph_batch_table_bytes", "tsph_batch_prepare", "tsph_batch_sort", "tsph_batch_density_alpha",
     "tsph_batch_solver_begin", "tsph_batch_solver_begin_signal", "tsph_batch_solver_psi",
@@ -174,6 +174,7 @@ def load(path=None):
         "tsph_solver_psi_taped": [P, O, O, C, _i32, _vp, _vp, _vp, sz, _vp],
         "tsph_solver_psi_frozen": [P, O, O, C, _i32, _vp, _vp, sz, _vp],
         "tsph_relist": [P, O, O, C, _vp, sz, _vp],
+        "tsph_relist_bound": [P, O, O, C, _vp, sz, _vp],
         "tsph_adj_begin": [P, O, C, A, _vp, _vp, _vp, _vp],
         "tsph_adj_solver_iteration": [P, O, O, C, A, _vp, sz, _vp],
         "tsph_adj_advect": [P, O, C, A, _vp, sz, _vp],

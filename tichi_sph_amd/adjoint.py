@@ -24,8 +24,12 @@ the gradients do not depend on the budget; the budget only trades HBM for
 the replay work.  A tape holds the sorted positions, masses, rest
 densities, pids and cell table, the densities, alpha factors, volumes and
 alpha_1 of pass A, vel_adv and kappa of every Jacobi iteration, and the
-advection's velocity and normals.  Neighbour lists are not stored:
-`tsph_relist` recomputes them bit-identically from the recorded positions.
+advection's velocity and normals, and (`DiffRunner.tape_lists`, on by
+default) the fluid neighbour lists: a taped step's pass A writes them into
+a list buffer of the tape's own (list_cap_f rows of the capacity, 4 bytes
+each), so they cost no copy.  The reverse then rebuilds only the boundary
+lists (`tsph_relist_bound`); without stored lists `tsph_relist` recomputes
+all of them bit-identically from the recorded positions.
 
     runner = DiffRunner(sim)
     x, v = rollout(runner, v0, mu, gamma, steps=200, every=20)
@@ -43,15 +47,19 @@ class StepTape:
     """Everything the reverse of one step reads that is not recomputed."""
 
     __slots__ = ("pos", "mass", "rest_density", "pid", "cell_start", "density", "alpha", "vol",
-                 "alpha_1", "vel", "normal", "iters", "calls")
+                 "alpha_1", "vel", "normal", "iters", "calls", "nbr_f", "nbr_nf")
 
     def __init__(self):
         self.iters = ([], [])  # per solve: [(vel_adv entering it, kappa), ...]
+        # the fluid lists (list_cap_f slot-major rows of the object's
+        # capacity) and their counts, when the runner stores them
+        self.nbr_f = self.nbr_nf = None
 
     def nbytes(self):
         ts = [self.pos, self.mass, self.rest_density, self.pid, self.cell_start, self.density, self.alpha,
               self.vol, self.alpha_1, self.vel, self.normal]
         ts += [t for s in self.iters for pair in s for t in pair]
+        ts += [t for t in (self.nbr_f, self.nbr_nf) if t is not None]
         return sum(t.numel() * t.element_size() for t in ts)
 
 
@@ -85,6 +93,9 @@ class DiffRunner:
         self.adj.grad = self.grad.data_ptr()
         # False skips the surface-tension reverse; allowed only while gamma == 0
         self.surface_tension_grad = True
+        # True stores each taped step's fluid lists (HBM for the relist's work)
+        self.tape_lists = True
+        self._own_lists = (f.t["nbr_f"], f.t["nbr_nf"])  # the Simulation's own list buffers
         self.force = None     # optional list of (div_iter, incom_iter) per step to replay
         self.step_index = 0
         self.iter_log = []    # (div_iter, incom_iter) of every forward step since the last reset
@@ -166,12 +177,21 @@ class DiffRunner:
         force = self.force[self.step_index] if self.force is not None else (None, None)
         if sim.bound_needs_sort():
             sim._sort_bound()
+        # pass A writes the fluid lists into the buffers this step's tape
+        # keeps, or into the Simulation's own (never into a kept tape's)
+        if tape and self.tape_lists:
+            lists = (torch.empty_like(self._own_lists[0]), torch.empty_like(self._own_lists[1]))
+        else:
+            lists = self._own_lists
+        self._use_lists(*lists)
         sim._call("sort", sim.lib.tsph_sort_object, P, ctypes.byref(f.struct()), sim.ctrl_ptr(), 0, ws, wsb, stp)
         f.adopt_swaps()
         tp = None
         n = f.part_num
         if tape:
             tp = StepTape()
+            if self.tape_lists:
+                tp.nbr_f, tp.nbr_nf = lists
             tp.pos = f.t["pos"][:n].clone()
             tp.mass = f.t["mass"][:n].clone()
             tp.rest_density = f.t["rest_density"][:n].clone()
@@ -196,6 +216,9 @@ class DiffRunner:
         sim._call("tail", sim.lib.tsph_tail, P, fs, sim.ctrl_ptr(), 0, ws, wsb, stp)
         if tape:
             tp.calls = sim.calls  # no later call: lists and density / alpha still resident
+        # the Simulation keeps its own list buffers between steps, so that no
+        # later step writes into a tape's
+        self._use_lists(*self._own_lists)
         self.iter_log.append((d_it, i_it))
         self.step_index += 1
         return tp
@@ -220,6 +243,8 @@ class DiffRunner:
         f.t["alpha"][:n].copy_(tp.alpha)
         f.t["vol"][:n].copy_(tp.vol)
         f.t["alpha_1"][:n].copy_(tp.alpha_1)
+        if tp.nbr_f is not None:
+            self._use_lists(tp.nbr_f, tp.nbr_nf)
         fs, bs = ctypes.byref(f.struct()), ctypes.byref(b.struct())
         A = ctypes.byref(self.adj)
         vp = ctypes.c_void_p
@@ -227,7 +252,10 @@ class DiffRunner:
             return pairs[-1][1].data_ptr() if pairs else None
 
         if tp.calls != sim.calls:
-            sim._call("relist", self.lib.tsph_relist, P, fs, bs, sim.ctrl_ptr(), ws, wsb, stp)
+            if tp.nbr_f is None:
+                sim._call("relist", self.lib.tsph_relist, P, fs, bs, sim.ctrl_ptr(), ws, wsb, stp)
+            else:
+                sim._call("relist_bound", self.lib.tsph_relist_bound, P, fs, bs, sim.ctrl_ptr(), ws, wsb, stp)
         self.adj.kappa_next = last_kappa(tp.iters[1])
         sim._call("adj_begin", self.lib.tsph_adj_begin, P, fs, sim.ctrl_ptr(), A, vp(tp.pid.data_ptr()),
                   vp(xbar_ref.data_ptr()), vp(vbar_ref.data_ptr()), stp)
@@ -254,6 +282,16 @@ class DiffRunner:
         sim._call("adj_density_alpha", self.lib.tsph_adj_density_alpha, P, fs, bs, A, stp)
         sim._call("adj_end", self.lib.tsph_adj_end, P, fs, A, vp(tp.pid.data_ptr()),
                   vp(xbar_ref.data_ptr()), vp(vbar_ref.data_ptr()), stp)
+        self._use_lists(*self._own_lists)
+
+    def _use_lists(self, nbr_f, nbr_nf):
+        """Points the fluid's list buffers (tensors and descriptor) at these."""
+        f = self.sim.fluid
+        if f.t["nbr_f"] is nbr_f and f.t["nbr_nf"] is nbr_nf:
+            return
+        f.t["nbr_f"], f.t["nbr_nf"] = nbr_f, nbr_nf
+        f._refresh_struct()
+        f.layout_epoch += 1  # a SceneBatch holding the old descriptor rebuilds it
 
     def _swap_v(self):
         self.buf["vbar"], self.buf["vbar_out"] = self.buf["vbar_out"], self.buf["vbar"]

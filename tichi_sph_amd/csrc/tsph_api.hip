@@ -38,7 +38,7 @@ template <class T>
 int solver_half(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, int, void*, void*,
                 const void*, void*, size_t, hipStream_t);
 template <class T>
-int relist(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
+int relist(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t, int);
 template <class T>
 int map_velocity(const TsphParams&, const TsphObject*, const int*, double, void*, hipStream_t);
 template <class T>
@@ -352,7 +352,17 @@ int tsph_relist(const TsphParams* p, TsphObject* f, TsphObject* b, TsphControl* 
   if (int rc = check_obj(b, "tsph_relist")) return rc;
   TSPH_REQUIRE(p->list_cap_f >= 1 && p->list_cap_b >= 1, TSPH_EINVAL,
                "tsph_relist: list capacities must be positive");
-  return DISPATCH(p, relist, *p, f, b, ctrl, ws, wsb, (hipStream_t)stream);
+  return DISPATCH(p, relist, *p, f, b, ctrl, ws, wsb, (hipStream_t)stream, 3);
+}
+
+int tsph_relist_bound(const TsphParams* p, TsphObject* f, TsphObject* b, TsphControl* ctrl,
+                      void* ws, size_t wsb, void* stream) {
+  if (int rc = check_params(p)) return rc;
+  if (int rc = check_obj(f, "tsph_relist_bound")) return rc;
+  if (int rc = check_obj(b, "tsph_relist_bound")) return rc;
+  TSPH_REQUIRE(p->list_cap_b >= 1, TSPH_EINVAL,
+               "tsph_relist_bound: boundary list capacity must be positive");
+  return DISPATCH(p, relist, *p, f, b, ctrl, ws, wsb, (hipStream_t)stream, 2);
 }
 
 int tsph_advect(const TsphParams* p, TsphObject* f, TsphObject* b, TsphControl* ctrl, void* ws,
