@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define TSPH_ABI_VERSION 4
+#define TSPH_ABI_VERSION 5
 #define TSPH_MAX_PHASE 8
 #define TSPH_MAX_ITER_LOG 256
 
@@ -368,6 +368,9 @@ typedef struct TsphAdjoint {
   void *tape_vel;     /* V4: recorded velocity after the advection (its w lane: X / psi) */
   void *tape_normal;  /* V4: recorded surface-tension normals */
   double *grad;       /* [2]: accumulated d loss / d dynamic_viscosity, d surface_tension_gamma */
+  void *bmom;         /* V4 [3][capacity]: each fluid particle's boundary sums of the step
+                         being reversed (tsph_adj_bound_sums) */
+  int32_t *bflag;     /* [capacity]: 1 where a particle has boundary neighbours, else 0 */
 } TsphAdjoint;
 
 /* tsph_solver_psi that also records vel_adv entering iteration k and the
@@ -394,11 +397,22 @@ int tsph_solver_psi_frozen(const TsphParams* p, TsphObject* fluid, TsphObject* b
 int tsph_relist(const TsphParams* p, TsphObject* fluid, TsphObject* bound, TsphControl* ctrl,
                 void* workspace, size_t ws_bytes, void* stream);
 
-/* tsph_relist for the boundary lists (nbr_b, nbr_nb) alone, for a caller
- * that stored the step's fluid lists (nbr_f, nbr_nf) and points the fluid
- * descriptor at that copy. */
-int tsph_relist_bound(const TsphParams* p, TsphObject* fluid, TsphObject* bound,
-                      TsphControl* ctrl, void* workspace, size_t ws_bytes, void* stream);
+/* The boundary neighbours' sums that the reverse of the Jacobi iterations
+ * and of pass A use in place of the (f,b) pairs (boundary velocities and
+ * kappa are zero, so those pairs depend on the boundary only through them):
+ * for each fluid particle i, over the boundary particles b in range
+ * (0 < r^2 <= r2_max, the lists' test), with x = x_i - x_b, X_b = pos.w,
+ * g = W'(r)/r and hq = (W'' - g)/r^2:
+ *   bmom[i]            = (sum X_b g x, sum X_b g)
+ *   bmom[capacity + i] = sum X_b hq (xx, yy, zz, xy)
+ *   bmom[2capacity+i]  = sum X_b hq (xz, yz, 0, 0)
+ * and bflag[i] = 1 if any b is in range.  Walks the boundary cells (not the
+ * lists) of the particles whose boundary-list count nbr_nb is not 0, so the
+ * fluid descriptor must hold the step's counts (as tsph_density_alpha or
+ * tsph_relist left them).  Call it once per reversed step, before
+ * tsph_adj_begin. */
+int tsph_adj_bound_sums(const TsphParams* p, TsphObject* fluid, TsphObject* bound,
+                        TsphAdjoint* adj, void* stream);
 
 /* Opens the reverse of one step (tail, SPH_update_pos sph.py:265-269):
  * reads d loss / d pos and d loss / d vel after the step from V4 arrays in

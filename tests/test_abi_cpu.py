@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     assert set(decl) == set(_lib.EXPORTED)
     for name in decl:
         assert hasattr(lib, name), name
-    assert lib.tsph_abi_version() == _lib.ABI_VERSION == 4
+    assert lib.tsph_abi_version() == _lib.ABI_VERSION == 5
 
 
 def test_struct_layout_matches_header(tmp_path):

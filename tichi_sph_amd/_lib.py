@@ -9,7 +9,7 @@ import os
 
 TSPH_MAX_PHASE = 8
 TSPH_MAX_ITER_LOG = 256
-ABI_VERSION = 4  # TSPH_ABI_VERSION of include/tichi_sph.h
+ABI_VERSION = 5  # TSPH_ABI_VERSION of include/tichi_sph.h
 TSPH_FLAG_FAST_QUOTIENT = 1  # TsphParams.flags
 TSPH_FLAG_ST_RECORDS = 2
 TSPH_OBJ_STATIC_SUMS = 1  # TsphObject.flags
@@ -78,7 +78,7 @@ class TsphControl(ctypes.Structure):
 
 # Reference kernel ids (sph.py), include/tichi_sph.h
 _ADJ_PTRS = ["xbar", "vbar", "vbar_out", "rhobar", "alphabar", "nbar", "a1bar",
-             "tape_vel_adv", "kappa_next", "tape_vel", "tape_normal", "grad"]
+             "tape_vel_adv", "kappa_next", "tape_vel", "tape_normal", "grad", "bmom", "bflag"]
 
 
 class TsphAdjoint(ctypes.Structure):
@@ -108,7 +108,7 @@ EXPORTED = [
     "tsph_verify_fast_quotient",
     "tsph_solver_iteration", "tsph_solver_psi", "tsph_solver_update", "tsph_advect",
     "tsph_fbm_round", "tsph_tail",
-    "tsph_ref_kernel", "tsph_solver_psi_taped", "tsph_solver_psi_frozen", "tsph_relist", "tsph_relist_bound", "tsph_adj_begin",
+    "tsph_ref_kernel", "tsph_solver_psi_taped", "tsph_solver_psi_frozen", "tsph_relist", "tsph_adj_bound_sums", "tsph_adj_begin",
     "tsph_adj_solver_iteration", "tsph_adj_advect", "tsph_adj_density_alpha", "tsph_adj_end",
     "tsph_batch_table_bytes", "tsph_batch_prepare", "tsph_batch_sort", "tsph_batch_density_alpha",
     "tsph_batch_solver_begin", "tsph_batch_solver_begin_signal", "tsph_batch_solver_psi",
@@ -174,11 +174,11 @@ def load(path=None):
         "tsph_solver_psi_taped": [P, O, O, C, _i32, _vp, _vp, _vp, sz, _vp],
         "tsph_solver_psi_frozen": [P, O, O, C, _i32, _vp, _vp, sz, _vp],
         "tsph_relist": [P, O, O, C, _vp, sz, _vp],
-        "tsph_relist_bound": [P, O, O, C, _vp, sz, _vp],
         "tsph_adj_begin": [P, O, C, A, _vp, _vp, _vp, _vp],
         "tsph_adj_solver_iteration": [P, O, O, C, A, _vp, sz, _vp],
         "tsph_adj_advect": [P, O, C, A, _vp, sz, _vp],
         "tsph_adj_density_alpha": [P, O, O, A, _vp],
+        "tsph_adj_bound_sums": [P, O, O, A, _vp],
         "tsph_adj_end": [P, O, A, _vp, _vp, _vp, _vp],
         "tsph_batch_prepare": [P, O, O, _vp, _vp, _vp, _i32, _vp, _vp, sz, BI, _vp],
         "tsph_batch_sort": [BI, _vp, O, _i32, _i32, _vp],

@@ -27,9 +27,11 @@ alpha_1 of pass A, vel_adv and kappa of every Jacobi iteration, and the
 advection's velocity and normals, and (`DiffRunner.tape_lists`, on by
 default) the fluid neighbour lists: a taped step's pass A writes them into
 a list buffer of the tape's own (list_cap_f rows of the capacity, 4 bytes
-each), so they cost no copy.  The reverse then rebuilds only the boundary
-lists (`tsph_relist_bound`); without stored lists `tsph_relist` recomputes
-all of them bit-identically from the recorded positions.
+each, with both lists' counts), so they cost no copy; without stored lists `tsph_relist`
+recomputes them bit-identically from the recorded positions.  The boundary
+enters the reverse only through per-particle sums over the boundary
+neighbours (`tsph_adj_bound_sums`, recomputed for each reversed step), so
+the boundary lists are neither stored nor rebuilt.
 
     runner = DiffRunner(sim)
     x, v = rollout(runner, v0, mu, gamma, steps=200, every=20)
@@ -47,19 +49,20 @@ class StepTape:
     """Everything the reverse of one step reads that is not recomputed."""
 
     __slots__ = ("pos", "mass", "rest_density", "pid", "cell_start", "density", "alpha", "vol",
-                 "alpha_1", "vel", "normal", "iters", "calls", "nbr_f", "nbr_nf")
+                 "alpha_1", "vel", "normal", "iters", "calls", "nbr_f", "nbr_nf", "nbr_nb")
 
     def __init__(self):
         self.iters = ([], [])  # per solve: [(vel_adv entering it, kappa), ...]
         # the fluid lists (list_cap_f slot-major rows of the object's
-        # capacity) and their counts, when the runner stores them
-        self.nbr_f = self.nbr_nf = None
+        # capacity), their counts and the boundary lists' counts, when the
+        # runner stores them
+        self.nbr_f = self.nbr_nf = self.nbr_nb = None
 
     def nbytes(self):
         ts = [self.pos, self.mass, self.rest_density, self.pid, self.cell_start, self.density, self.alpha,
               self.vol, self.alpha_1, self.vel, self.normal]
         ts += [t for s in self.iters for pair in s for t in pair]
-        ts += [t for t in (self.nbr_f, self.nbr_nf) if t is not None]
+        ts += [t for t in (self.nbr_f, self.nbr_nf, self.nbr_nb) if t is not None]
         return sum(t.numel() * t.element_size() for t in ts)
 
 
@@ -86,6 +89,8 @@ class DiffRunner:
                     for name in ("xbar", "vbar", "vbar_out", "nbar", "a1bar")}
         for name in ("rhobar", "alphabar"):
             self.buf[name] = torch.zeros(cap, dtype=dt_, device=dev)
+        self.buf["bmom"] = torch.zeros(3 * cap, 4, dtype=dt_, device=dev)  # boundary sums
+        self.buf["bflag"] = torch.zeros(cap, dtype=torch.int32, device=dev)
         self.grad = torch.zeros(2, dtype=torch.float64, device=dev)
         self.adj = _lib.TsphAdjoint()
         for name, t in self.buf.items():
@@ -95,7 +100,7 @@ class DiffRunner:
         self.surface_tension_grad = True
         # True stores each taped step's fluid lists (HBM for the relist's work)
         self.tape_lists = True
-        self._own_lists = (f.t["nbr_f"], f.t["nbr_nf"])  # the Simulation's own list buffers
+        self._own_lists = (f.t["nbr_f"], f.t["nbr_nf"], f.t["nbr_nb"])  # the Simulation's own
         self.force = None     # optional list of (div_iter, incom_iter) per step to replay
         self.step_index = 0
         self.iter_log = []    # (div_iter, incom_iter) of every forward step since the last reset
@@ -180,7 +185,7 @@ class DiffRunner:
         # pass A writes the fluid lists into the buffers this step's tape
         # keeps, or into the Simulation's own (never into a kept tape's)
         if tape and self.tape_lists:
-            lists = (torch.empty_like(self._own_lists[0]), torch.empty_like(self._own_lists[1]))
+            lists = tuple(torch.empty_like(t) for t in self._own_lists)
         else:
             lists = self._own_lists
         self._use_lists(*lists)
@@ -191,7 +196,7 @@ class DiffRunner:
         if tape:
             tp = StepTape()
             if self.tape_lists:
-                tp.nbr_f, tp.nbr_nf = lists
+                tp.nbr_f, tp.nbr_nf, tp.nbr_nb = lists
             tp.pos = f.t["pos"][:n].clone()
             tp.mass = f.t["mass"][:n].clone()
             tp.rest_density = f.t["rest_density"][:n].clone()
@@ -244,18 +249,16 @@ class DiffRunner:
         f.t["vol"][:n].copy_(tp.vol)
         f.t["alpha_1"][:n].copy_(tp.alpha_1)
         if tp.nbr_f is not None:
-            self._use_lists(tp.nbr_f, tp.nbr_nf)
+            self._use_lists(tp.nbr_f, tp.nbr_nf, tp.nbr_nb)
         fs, bs = ctypes.byref(f.struct()), ctypes.byref(b.struct())
         A = ctypes.byref(self.adj)
         vp = ctypes.c_void_p
         def last_kappa(pairs):
             return pairs[-1][1].data_ptr() if pairs else None
 
-        if tp.calls != sim.calls:
-            if tp.nbr_f is None:
-                sim._call("relist", self.lib.tsph_relist, P, fs, bs, sim.ctrl_ptr(), ws, wsb, stp)
-            else:
-                sim._call("relist_bound", self.lib.tsph_relist_bound, P, fs, bs, sim.ctrl_ptr(), ws, wsb, stp)
+        if tp.calls != sim.calls and tp.nbr_f is None:
+            sim._call("relist", self.lib.tsph_relist, P, fs, bs, sim.ctrl_ptr(), ws, wsb, stp)
+        sim._call("adj_bound_sums", self.lib.tsph_adj_bound_sums, P, fs, bs, A, stp)
         self.adj.kappa_next = last_kappa(tp.iters[1])
         sim._call("adj_begin", self.lib.tsph_adj_begin, P, fs, sim.ctrl_ptr(), A, vp(tp.pid.data_ptr()),
                   vp(xbar_ref.data_ptr()), vp(vbar_ref.data_ptr()), stp)
@@ -284,12 +287,13 @@ class DiffRunner:
                   vp(xbar_ref.data_ptr()), vp(vbar_ref.data_ptr()), stp)
         self._use_lists(*self._own_lists)
 
-    def _use_lists(self, nbr_f, nbr_nf):
-        """Points the fluid's list buffers (tensors and descriptor) at these."""
+    def _use_lists(self, nbr_f, nbr_nf, nbr_nb):
+        """Points the fluid's list buffers (tensors and descriptor) at these:
+        the fluid lists, their counts and the boundary lists' counts."""
         f = self.sim.fluid
-        if f.t["nbr_f"] is nbr_f and f.t["nbr_nf"] is nbr_nf:
+        if f.t["nbr_f"] is nbr_f and f.t["nbr_nf"] is nbr_nf and f.t["nbr_nb"] is nbr_nb:
             return
-        f.t["nbr_f"], f.t["nbr_nf"] = nbr_f, nbr_nf
+        f.t["nbr_f"], f.t["nbr_nf"], f.t["nbr_nb"] = nbr_f, nbr_nf, nbr_nb
         f._refresh_struct()
         f.layout_epoch += 1  # a SceneBatch holding the old descriptor rebuilds it
 

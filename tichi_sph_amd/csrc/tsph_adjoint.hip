@@ -89,6 +89,77 @@ __device__ __forceinline__ void add_J(T& ax, T& ay, T& az, const D2<T>& d, T dx,
 }
 
 // ---------------------------------------------------------------------------
+// Boundary sums.  Boundary particles carry no velocity, kappa or adjoint, so
+// in the reverse of the Jacobi iterations and of pass A every (f,b) pair of
+// particle l enters as  X_b G(x_lb) . a  and  J(x_lb) (X_b c)  with a, c
+// depending on l alone.  Summed over b these are  a . A_l  and  M_l c  with
+//   A_l = sum_b X_b g x,   M_l = sum_b X_b J(x) = G_l I + H_l,
+//   G_l = sum_b X_b g,     H_l = sum_b X_b hq x x^T,
+// so the reverse sweeps read three records instead of walking the boundary.
+// ---------------------------------------------------------------------------
+constexpr int kBsumU = 8;  // boundary candidates in flight per lane
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_adj_bound_sums(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
+                     const int* __restrict__ nbr_nb, const V4<T>* __restrict__ bpos,
+                     const int* __restrict__ bstart, V4<T>* __restrict__ bmom,
+                     int* __restrict__ bflag) {
+  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  if (nbr_nb[i] == 0) {  // the step found no boundary neighbour: nothing to walk
+    bflag[i] = 0;
+    return;
+  }
+  const V4<T> p = fpos[i];
+  int c[3];
+  node_encode(k, p.x, p.y, p.z, c);
+  T ax = 0, ay = 0, az = 0, G = 0, hxx = 0, hyy = 0, hzz = 0, hxy = 0, hxz = 0, hyz = 0;
+  int any = 0;
+  walk_cells_batched<kBsumU>(k, bstart, c, bpos, [&](int j, const V4<T>& q) {
+    const T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    const T r2 = s_r2(dx, dy, dz);
+    if (!(r2 <= k.r2max && r2 > T(0))) return;  // the boundary lists' test
+    const D2<T> e = kern_d2(k, r2);
+    const T xg = q.w * e.g, xh = q.w * e.hq;
+    ax += xg * dx;
+    ay += xg * dy;
+    az += xg * dz;
+    G += xg;
+    hxx += xh * dx * dx;
+    hyy += xh * dy * dy;
+    hzz += xh * dz * dz;
+    hxy += xh * dx * dy;
+    hxz += xh * dx * dz;
+    hyz += xh * dy * dz;
+    any = 1;
+  });
+  bflag[i] = any;
+  if (any) {
+    bmom[i] = v4(ax, ay, az, G);
+    bmom[(long)cap + i] = v4(hxx, hyy, hzz, hxy);
+    bmom[2L * cap + i] = v4(hxz, hyz, T(0), T(0));
+  }
+}
+
+// The boundary sums of particle i: A (xyz of m0) and M c = G c + H c.
+template <class T>
+struct BoundSums {
+  V4<T> m0, m1, m2;
+  __device__ __forceinline__ T dot_A(T x, T y, T z) const { return m0.x * x + m0.y * y + m0.z * z; }
+  __device__ __forceinline__ void add_M(T& ox, T& oy, T& oz, T cx, T cy, T cz) const {
+    ox += m0.w * cx + (m1.x * cx + m1.w * cy + m2.x * cz);
+    oy += m0.w * cy + (m1.w * cx + m1.y * cy + m2.y * cz);
+    oz += m0.w * cz + (m2.x * cx + m2.y * cy + m1.z * cz);
+  }
+};
+template <class T>
+__device__ __forceinline__ BoundSums<T> load_bound_sums(const V4<T>* __restrict__ bmom, int cap,
+                                                        int i) {
+  return BoundSums<T>{bmom[i], bmom[(long)cap + i], bmom[2L * cap + i]};
+}
+
+// ---------------------------------------------------------------------------
 // Tail reverse: pos' = pos + dt vel_adv, vel' = vel_adv (sph.py:259-269).
 // ---------------------------------------------------------------------------
 template <class T>
@@ -138,10 +209,9 @@ template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_adj_kappa(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
                 const T* __restrict__ alpha, const V4<T>* __restrict__ vbar,
-                const int* __restrict__ fstart, const V4<T>* __restrict__ bpos,
-                const int* __restrict__ bstart, const int* __restrict__ nbr_f,
-                const int* __restrict__ nbr_b, const int* __restrict__ nbr_nf,
-                const int* __restrict__ nbr_nb, V4<T>* __restrict__ tvel, T* __restrict__ rhobar,
+                const int* __restrict__ fstart, const int* __restrict__ nbr_f,
+                const int* __restrict__ nbr_nf, const V4<T>* __restrict__ bmom,
+                const int* __restrict__ bflag, V4<T>* __restrict__ tvel, T* __restrict__ rhobar,
                 T* __restrict__ alphabar, V4<T>* __restrict__ xbar, const TsphControl* ctrl) {
   const int base = xcd_block(blockIdx.x, gridDim.x) * kBlock;
   const int cl = coop_lane<kCoop>();
@@ -161,6 +231,9 @@ __global__ void __launch_bounds__(kBlock)
   const T al = alpha[i];
   const T rb0 = rhobar[i], ab0 = alphabar[i];
   const V4<T> xb = xbar[i];
+  const bool hb = bflag[i] != 0;
+  BoundSums<T> bs{};
+  if (hb) bs = load_bound_sums(bmom, cap, i);
   T kb = 0, ax = 0, ay = 0, az = 0;
   auto term = [&](const Rec2<T>& d, bool fluid) {
     const V4<T>& q = d.a;   // q.w = X_j = m_j
@@ -173,16 +246,21 @@ __global__ void __launch_bounds__(kBlock)
     T c = ki * q.w + uj.w * Xi;
     add_J(ax, ay, az, e, dx, dy, dz, c * mx, c * my, c * mz);
   };
+  // (f,f) pairs; the (f,b) pairs (u_b = kappa_b = 0) are w_l . A_l and
+  // kappa_l M_l w_l from the boundary sums
   sweep_lists_coop<kCoop, T, Rec2<T>>(
-      k, cap, i, cl, p, fstart, bstart, fpos, bpos, nbr_f, nbr_b, nbr_nf[i], nbr_nb[i],
-      [&](int j) { return Rec2<T>{fpos[j], vbar[j]}; },
-      [&](int j) { return Rec2<T>{bpos[j], v4(T(0), T(0), T(0), T(0))}; },
-      [&](const Rec2<T>& d) { term(d, true); }, [&](const Rec2<T>& d) { term(d, false); });
+      k, cap, i, cl, p, fstart, nullptr, fpos, nullptr, nbr_f, nullptr, nbr_nf[i], 0,
+      [&](int j) { return Rec2<T>{fpos[j], vbar[j]}; }, [&](int) { return Rec2<T>{}; },
+      [&](const Rec2<T>& d) { term(d, true); }, [&](const Rec2<T>&) {});
   kb = group_sum<kCoop>(kb);
   ax = group_sum<kCoop>(ax);
   ay = group_sum<kCoop>(ay);
   az = group_sum<kCoop>(az);
   if (cl != 0) continue;
+  if (hb) {
+    kb += bs.dot_A(wx, wy, wz);
+    bs.add_M(ax, ay, az, ki * wx, ki * wy, ki * wz);
+  }
   const T s = -(T(1) / dt);
   kb *= s;
   const T pb = ki > T(0) ? kb / al : T(0);
@@ -205,9 +283,8 @@ __global__ void __launch_bounds__(kBlock)
     k_adj_vel(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
               const V4<T>* __restrict__ tvel, const V4<T>* __restrict__ vbar,
               const T* __restrict__ kappa_next, const int* __restrict__ fstart,
-              const V4<T>* __restrict__ bpos, const int* __restrict__ bstart,
-              const int* __restrict__ nbr_f, const int* __restrict__ nbr_b,
-              const int* __restrict__ nbr_nf, const int* __restrict__ nbr_nb,
+              const int* __restrict__ nbr_f, const int* __restrict__ nbr_nf,
+              const V4<T>* __restrict__ bmom, const int* __restrict__ bflag,
               V4<T>* __restrict__ vbar_out, V4<T>* __restrict__ xbar, const TsphControl* ctrl) {
   const int base = xcd_block(blockIdx.x, gridDim.x) * kBlock;
   const int cl = coop_lane<kCoop>();
@@ -225,6 +302,9 @@ __global__ void __launch_bounds__(kBlock)
   const V4<T> u = vbar[i];
   const T kn = kappa_next ? kappa_next[i] : T(0);
   const V4<T> xb = xbar[i];
+  const bool hb = bflag[i] != 0;
+  BoundSums<T> bs{};
+  if (hb) bs = load_bound_sums(bmom, cap, i);
   T vx = 0, vy = 0, vz = 0, ax = 0, ay = 0, az = 0;
   auto term = [&](const Rec2<T>& d) {
     const V4<T>& q = d.a;   // q.w = X_j
@@ -238,10 +318,12 @@ __global__ void __launch_bounds__(kBlock)
     vz += cg * dz;
     add_J(ax, ay, az, e, dx, dy, dz, c * (vi.x - vj.x), c * (vi.y - vj.y), c * (vi.z - vj.z));
   };
+  // (f,f) pairs; the (f,b) pairs (v_b = psib_b = 0) are psib_l A_l and
+  // psib_l M_l v_l from the boundary sums
   sweep_lists_coop<kCoop, T, Rec2<T>>(
-      k, cap, i, cl, p, fstart, bstart, fpos, bpos, nbr_f, nbr_b, nbr_nf[i], nbr_nb[i],
-      [&](int j) { return Rec2<T>{fpos[j], tvel[j]}; },
-      [&](int j) { return Rec2<T>{bpos[j], v4(T(0), T(0), T(0), T(0))}; }, term, term);
+      k, cap, i, cl, p, fstart, nullptr, fpos, nullptr, nbr_f, nullptr, nbr_nf[i], 0,
+      [&](int j) { return Rec2<T>{fpos[j], tvel[j]}; }, [&](int) { return Rec2<T>{}; }, term,
+      [&](const Rec2<T>&) {});
   vx = group_sum<kCoop>(vx);
   vy = group_sum<kCoop>(vy);
   vz = group_sum<kCoop>(vz);
@@ -249,6 +331,12 @@ __global__ void __launch_bounds__(kBlock)
   ay = group_sum<kCoop>(ay);
   az = group_sum<kCoop>(az);
   if (cl != 0) continue;
+  if (hb) {
+    vx += pbi * bs.m0.x;
+    vy += pbi * bs.m0.y;
+    vz += pbi * bs.m0.z;
+    bs.add_M(ax, ay, az, pbi * vi.x, pbi * vi.y, pbi * vi.z);
+  }
   vbar_out[i] = v4(u.x + dt * vx, u.y + dt * vy, u.z + dt * vz, kn);
   xbar[i] = v4(xb.x + dt * ax, xb.y + dt * ay, xb.z + dt * az);
   }
@@ -497,10 +585,9 @@ __global__ void __launch_bounds__(kBlock)
     k_adj_pass_a(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
                  const T* __restrict__ fmass, const T* __restrict__ rhobar,
                  const V4<T>* __restrict__ a1bar, const int* __restrict__ fstart,
-                 const V4<T>* __restrict__ bpos, const T* __restrict__ bmass,
-                 const int* __restrict__ bstart, const int* __restrict__ nbr_f,
-                 const int* __restrict__ nbr_b, const int* __restrict__ nbr_nf,
-                 const int* __restrict__ nbr_nb, V4<T>* __restrict__ xbar) {
+                 const int* __restrict__ nbr_f, const int* __restrict__ nbr_nf,
+                 const V4<T>* __restrict__ bmom, const int* __restrict__ bflag,
+                 V4<T>* __restrict__ xbar) {
   const int base = xcd_block(blockIdx.x, gridDim.x) * kBlock;
   const int cl = coop_lane<kCoop>();
 #pragma unroll 1
@@ -513,29 +600,29 @@ __global__ void __launch_bounds__(kBlock)
   const V4<T> abi = a1bar[i];
   const T xm_i = Xi * Xi / mi;
   const V4<T> xb = xbar[i];  // requested before the walk; only this group writes it
+  const bool hb = bflag[i] != 0;
+  BoundSums<T> bs{};
+  if (hb) bs = load_bound_sums(bmom, cap, i);
   T ax = 0, ay = 0, az = 0;
-  // record: a = position (w = X_j), b = a1bar_j (w = a2bar_j),
-  // c = (rho_bar_j, m_j, is_fluid)
+  // (f,f) pairs.  Record: a = position (w = X_j), b = a1bar_j (w = a2bar_j),
+  // c = (rho_bar_j, m_j).  The (f,b) pairs (rho_bar_b = 0, no alpha_2 term,
+  // DFSPH: m_b = X_b) are rho_bar_l A_l + M_l a1bar_l from the boundary sums.
   sweep_lists_coop<kCoop, T, Rec4<T>>(
-      k, cap, i, cl, p, fstart, bstart, fpos, bpos, nbr_f, nbr_b, nbr_nf[i], nbr_nb[i],
+      k, cap, i, cl, p, fstart, nullptr, fpos, nullptr, nbr_f, nullptr, nbr_nf[i], 0,
       [&](int j) {
         V4<T> q = fpos[j];
-        V4<T> extra = v4(rhobar[j], q.w, T(1), T(0));  // DFSPH: m_j = X_j
+        V4<T> extra = v4(rhobar[j], q.w, T(0), T(0));  // DFSPH: m_j = X_j
         return Rec4<T>{q, a1bar[j], extra, V4<T>{}};
       },
-      [&](int j) {
-        V4<T> extra = v4(T(0), bmass[j], T(0), T(0));
-        return Rec4<T>{bpos[j], V4<T>{}, extra, V4<T>{}};
-      },
+      [&](int) { return Rec4<T>{}; },
       [&](const Rec4<T>& d) {
         const V4<T>& q = d.a;
         const V4<T>& abj = d.b;
         const T rbj = d.c.x, mj = d.c.y;
-        const bool fluid = d.c.z != T(0);
         T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
         D2<T> e = kern_d2(k, dx * dx + dy * dy + dz * dz);
         T cd = (rbi * mj + rbj * mi) * e.g;
-        T c2 = fluid ? T(2) * e.g * e.w2 * (abi.w * q.w * q.w / mj + abj.w * xm_i) : T(0);
+        T c2 = T(2) * e.g * e.w2 * (abi.w * q.w * q.w / mj + abj.w * xm_i);
         T s1 = cd + c2;
         ax += s1 * dx;
         ay += s1 * dy;
@@ -543,21 +630,17 @@ __global__ void __launch_bounds__(kBlock)
         add_J(ax, ay, az, e, dx, dy, dz, q.w * abi.x - Xi * abj.x, q.w * abi.y - Xi * abj.y,
               q.w * abi.z - Xi * abj.z);
       },
-      [&](const Rec4<T>& d) {
-        const V4<T>& q = d.a;
-        const T mj = d.c.y;
-        T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-        D2<T> e = kern_d2(k, dx * dx + dy * dy + dz * dz);
-        T cd = rbi * mj * e.g;
-        ax += cd * dx;
-        ay += cd * dy;
-        az += cd * dz;
-        add_J(ax, ay, az, e, dx, dy, dz, q.w * abi.x, q.w * abi.y, q.w * abi.z);
-      });
+      [&](const Rec4<T>&) {});
   ax = group_sum<kCoop>(ax);
   ay = group_sum<kCoop>(ay);
   az = group_sum<kCoop>(az);
   if (cl != 0) continue;
+  if (hb) {
+    ax += rbi * bs.m0.x;
+    ay += rbi * bs.m0.y;
+    az += rbi * bs.m0.z;
+    bs.add_M(ax, ay, az, abi.x, abi.y, abi.z);
+  }
   xbar[i] = v4(xb.x + ax, xb.y + ay, xb.z + az);
   }
 }
@@ -596,16 +679,17 @@ int adj_solver_iteration(const TsphParams& prm, TsphObject* f, TsphObject* b, Ts
   KP<T> k = make_kp<T>(prm);
   if (f->n == 0) return 0;
   int nb = blocks_for(f->n);
+  (void)b;  // the boundary enters through the boundary sums
   hipLaunchKernelGGL(k_adj_kappa<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
                      (const V4<T>*)f->pos, (const T*)f->alpha, (const V4<T>*)a->vbar,
-                     f->cell_start, (const V4<T>*)b->pos, b->cell_start, f->nbr_f, f->nbr_b,
-                     f->nbr_nf, f->nbr_nb, (V4<T>*)a->tape_vel_adv, (T*)a->rhobar,
-                     (T*)a->alphabar, (V4<T>*)a->xbar, ctrl);
+                     f->cell_start, f->nbr_f, f->nbr_nf, (const V4<T>*)a->bmom, a->bflag,
+                     (V4<T>*)a->tape_vel_adv, (T*)a->rhobar, (T*)a->alphabar, (V4<T>*)a->xbar,
+                     ctrl);
   hipLaunchKernelGGL(k_adj_vel<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
                      (const V4<T>*)f->pos, (const V4<T>*)a->tape_vel_adv, (const V4<T>*)a->vbar,
-                     (const T*)a->kappa_next, f->cell_start, (const V4<T>*)b->pos, b->cell_start,
-                     f->nbr_f, f->nbr_b, f->nbr_nf, f->nbr_nb, (V4<T>*)a->vbar_out,
-                     (V4<T>*)a->xbar, ctrl);
+                     (const T*)a->kappa_next, f->cell_start, f->nbr_f, f->nbr_nf,
+                     (const V4<T>*)a->bmom, a->bflag, (V4<T>*)a->vbar_out, (V4<T>*)a->xbar,
+                     ctrl);
   TSPH_LAUNCH_CHECK("adj_solver_iteration");
   return 0;
 }
@@ -651,10 +735,21 @@ int adj_density_alpha(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphA
                      (V4<T>*)a->a1bar);
   hipLaunchKernelGGL(k_adj_pass_a<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
                      (const V4<T>*)f->pos, (const T*)f->mass, (const T*)a->rhobar,
-                     (const V4<T>*)a->a1bar, f->cell_start, (const V4<T>*)b->pos,
-                     (const T*)b->mass, b->cell_start, f->nbr_f, f->nbr_b, f->nbr_nf, f->nbr_nb,
-                     (V4<T>*)a->xbar);
+                     (const V4<T>*)a->a1bar, f->cell_start, f->nbr_f, f->nbr_nf,
+                     (const V4<T>*)a->bmom, a->bflag, (V4<T>*)a->xbar);
   TSPH_LAUNCH_CHECK("adj_density_alpha");
+  return 0;
+}
+
+template <class T>
+int adj_bound_sums(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphAdjoint* a,
+                   hipStream_t st) {
+  KP<T> k = make_kp<T>(prm);
+  if (f->n == 0) return 0;
+  hipLaunchKernelGGL(k_adj_bound_sums<T>, dim3(blocks_for(f->n)), dim3(kBlock), 0, st, k, f->n,
+                     f->capacity, (const V4<T>*)f->pos, f->nbr_nb, (const V4<T>*)b->pos,
+                     b->cell_start, (V4<T>*)a->bmom, a->bflag);
+  TSPH_LAUNCH_CHECK("adj_bound_sums");
   return 0;
 }
 
@@ -668,7 +763,9 @@ int adj_density_alpha(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphA
   template int adj_advect<T>(const TsphParams&, TsphObject*, TsphControl*, TsphAdjoint*, void*, \
                              size_t, hipStream_t);                                              \
   template int adj_density_alpha<T>(const TsphParams&, TsphObject*, TsphObject*, TsphAdjoint*,  \
-                                    hipStream_t);
+                                    hipStream_t);                                               \
+  template int adj_bound_sums<T>(const TsphParams&, TsphObject*, TsphObject*, TsphAdjoint*,     \
+                                 hipStream_t);
 TSPH_INST(float)
 TSPH_INST(double)
 #undef TSPH_INST

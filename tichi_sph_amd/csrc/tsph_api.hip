@@ -38,7 +38,7 @@ template <class T>
 int solver_half(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, int, int, void*, void*,
                 const void*, void*, size_t, hipStream_t);
 template <class T>
-int relist(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t, int);
+int relist(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template <class T>
 int map_velocity(const TsphParams&, const TsphObject*, const int*, double, void*, hipStream_t);
 template <class T>
@@ -110,6 +110,8 @@ int adj_advect(const TsphParams&, TsphObject*, TsphControl*, TsphAdjoint*, void*
                hipStream_t);
 template <class T>
 int adj_density_alpha(const TsphParams&, TsphObject*, TsphObject*, TsphAdjoint*, hipStream_t);
+template <class T>
+int adj_bound_sums(const TsphParams&, TsphObject*, TsphObject*, TsphAdjoint*, hipStream_t);
 template <class T>
 int advect(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template <class T>
@@ -186,6 +188,12 @@ static int check_adj(const TsphParams* p, const TsphAdjoint* a, const char* what
   TSPH_REQUIRE(a, TSPH_EINVAL, "%s: null adjoint", what);
   TSPH_REQUIRE(p->solver_type == TSPH_DFSPH && p->phase_num == 1, TSPH_EINVAL,
                "%s: the adjoint supports one-phase DFSPH only", what);
+  return 0;
+}
+
+// The entries that read or write the boundary sums (tsph_adj_bound_sums).
+static int check_bsums(const TsphAdjoint* a, const char* what) {
+  TSPH_REQUIRE(a->bmom && a->bflag, TSPH_EINVAL, "%s: null boundary sums (bmom, bflag)", what);
   return 0;
 }
 
@@ -352,17 +360,7 @@ int tsph_relist(const TsphParams* p, TsphObject* f, TsphObject* b, TsphControl* 
   if (int rc = check_obj(b, "tsph_relist")) return rc;
   TSPH_REQUIRE(p->list_cap_f >= 1 && p->list_cap_b >= 1, TSPH_EINVAL,
                "tsph_relist: list capacities must be positive");
-  return DISPATCH(p, relist, *p, f, b, ctrl, ws, wsb, (hipStream_t)stream, 3);
-}
-
-int tsph_relist_bound(const TsphParams* p, TsphObject* f, TsphObject* b, TsphControl* ctrl,
-                      void* ws, size_t wsb, void* stream) {
-  if (int rc = check_params(p)) return rc;
-  if (int rc = check_obj(f, "tsph_relist_bound")) return rc;
-  if (int rc = check_obj(b, "tsph_relist_bound")) return rc;
-  TSPH_REQUIRE(p->list_cap_b >= 1, TSPH_EINVAL,
-               "tsph_relist_bound: boundary list capacity must be positive");
-  return DISPATCH(p, relist, *p, f, b, ctrl, ws, wsb, (hipStream_t)stream, 2);
+  return DISPATCH(p, relist, *p, f, b, ctrl, ws, wsb, (hipStream_t)stream);
 }
 
 int tsph_advect(const TsphParams* p, TsphObject* f, TsphObject* b, TsphControl* ctrl, void* ws,
@@ -411,6 +409,7 @@ int tsph_adj_solver_iteration(const TsphParams* p, TsphObject* f, TsphObject* b,
   if (int rc = check_obj(f, "tsph_adj_solver_iteration")) return rc;
   if (int rc = check_obj(b, "tsph_adj_solver_iteration")) return rc;
   if (int rc = check_adj(p, adj, "tsph_adj_solver_iteration")) return rc;
+  if (int rc = check_bsums(adj, "tsph_adj_solver_iteration")) return rc;
   TSPH_REQUIRE(adj->tape_vel_adv, TSPH_EINVAL, "tsph_adj_solver_iteration: null tape");
   return DISPATCH(p, adj_solver_iteration, *p, f, b, ctrl, adj, ws, wsb, (hipStream_t)stream);
 }
@@ -427,12 +426,23 @@ int tsph_adj_advect(const TsphParams* p, TsphObject* f, TsphControl* ctrl, TsphA
   return DISPATCH(p, adj_advect, *p, f, ctrl, adj, ws, wsb, (hipStream_t)stream);
 }
 
+int tsph_adj_bound_sums(const TsphParams* p, TsphObject* f, TsphObject* b, TsphAdjoint* adj,
+                        void* stream) {
+  if (int rc = check_params(p)) return rc;
+  if (int rc = check_obj(f, "tsph_adj_bound_sums")) return rc;
+  if (int rc = check_obj(b, "tsph_adj_bound_sums")) return rc;
+  if (int rc = check_adj(p, adj, "tsph_adj_bound_sums")) return rc;
+  if (int rc = check_bsums(adj, "tsph_adj_bound_sums")) return rc;
+  return DISPATCH(p, adj_bound_sums, *p, f, b, adj, (hipStream_t)stream);
+}
+
 int tsph_adj_density_alpha(const TsphParams* p, TsphObject* f, TsphObject* b, TsphAdjoint* adj,
                            void* stream) {
   if (int rc = check_params(p)) return rc;
   if (int rc = check_obj(f, "tsph_adj_density_alpha")) return rc;
   if (int rc = check_obj(b, "tsph_adj_density_alpha")) return rc;
   if (int rc = check_adj(p, adj, "tsph_adj_density_alpha")) return rc;
+  if (int rc = check_bsums(adj, "tsph_adj_density_alpha")) return rc;
   return DISPATCH(p, adj_density_alpha, *p, f, b, adj, (hipStream_t)stream);
 }
 

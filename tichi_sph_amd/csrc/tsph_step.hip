@@ -1089,63 +1089,56 @@ int density_alpha(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphContr
 // without branches (per-branch increments get merged into an indexed
 // increment of a stack array).
 constexpr int kListsU = 8;
-constexpr int kListsFluid = 1, kListsBound = 2;  // which lists k_lists rebuilds
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_lists(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos, const int* __restrict__ fstart,
             const V4<T>* __restrict__ bpos, const int* __restrict__ bstart, int* __restrict__ nbr_f,
             int* __restrict__ nbr_b, int* __restrict__ nbr_nf, int* __restrict__ nbr_nb,
-            TsphControl* ctrl, int which) {
+            TsphControl* ctrl) {
   const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i >= n) return;
   const int capf = k.list_cap_f, capb = k.list_cap_b;
   const V4<T> p = fpos[i];
   int c[3];
   node_encode(k, p.x, p.y, p.z, c);
-  if (which & kListsFluid) {
-    int nf = 0;
-    walk_cells_batched<kListsU>(k, fstart, c, fpos, [&](int j, const V4<T>& q) {
-      const T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
-      const bool take = r2 <= k.r2max && r2 > T(0);
-      if (take && nf < capf) nbr_f[(long)nf * cap + i] = j;
-      nf += take ? 1 : 0;
-    });
-    if (nf > capf) {
-      atomicAdd(&ctrl->overflow_f, 1);
-      nf = capf + 1;
-    }
-    nbr_nf[i] = nf;
+  int nf = 0, nbb = 0;
+  walk_cells_batched<kListsU>(k, fstart, c, fpos, [&](int j, const V4<T>& q) {
+    const T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
+    const bool take = r2 <= k.r2max && r2 > T(0);
+    if (take && nf < capf) nbr_f[(long)nf * cap + i] = j;
+    nf += take ? 1 : 0;
+  });
+  walk_cells_batched<kListsU>(k, bstart, c, bpos, [&](int j, const V4<T>& q) {
+    const T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
+    const bool take = r2 <= k.r2max && r2 > T(0);
+    if (take && nbb < capb) nbr_b[(long)nbb * cap + i] = j;
+    nbb += take ? 1 : 0;
+  });
+  if (nf > capf) {
+    atomicAdd(&ctrl->overflow_f, 1);
+    nf = capf + 1;
   }
-  if (which & kListsBound) {
-    int nbb = 0;
-    walk_cells_batched<kListsU>(k, bstart, c, bpos, [&](int j, const V4<T>& q) {
-      const T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
-      const bool take = r2 <= k.r2max && r2 > T(0);
-      if (take && nbb < capb) nbr_b[(long)nbb * cap + i] = j;
-      nbb += take ? 1 : 0;
-    });
-    if (nbb > capb) {
-      atomicAdd(&ctrl->overflow_b, 1);
-      nbb = capb + 1;
-    }
-    nbr_nb[i] = nbb;
+  if (nbb > capb) {
+    atomicAdd(&ctrl->overflow_b, 1);
+    nbb = capb + 1;
   }
+  nbr_nf[i] = nf;
+  nbr_nb[i] = nbb;
 }
 
-// Rebuilds the fluid's neighbour lists (both, or the boundary lists alone)
-// from recorded sorted positions and cell table.  The adjoint recomputes
-// per-step lists this way where it does not store them (it stores the step's
-// densities and alpha factors).
+// Rebuilds the fluid's neighbour lists from recorded sorted positions and
+// cell table.  The adjoint recomputes per-step lists this way when it does
+// not store them (it stores the step's densities and alpha factors).
 template <class T>
 int relist(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl* ctrl, void* ws,
-           size_t wsb, hipStream_t st, int which) {
+           size_t wsb, hipStream_t st) {
   (void)ws;
   (void)wsb;
   KP<T> k = make_kp<T>(prm);
   if (f->n == 0) return 0;
   hipLaunchKernelGGL(k_lists<T>, dim3(blocks_for(f->n)), dim3(kBlock), 0, st, k, f->n, f->capacity,
                      (const V4<T>*)f->pos, f->cell_start, (const V4<T>*)b->pos, b->cell_start,
-                     f->nbr_f, f->nbr_b, f->nbr_nf, f->nbr_nb, ctrl, which);
+                     f->nbr_f, f->nbr_b, f->nbr_nf, f->nbr_nb, ctrl);
   TSPH_LAUNCH_CHECK("relist");
   return 0;
 }
@@ -1631,8 +1624,8 @@ TSPH_BATCH_INST(float)
 TSPH_BATCH_INST(double)
 #undef TSPH_BATCH_INST
 
-template int relist<float>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t, int);
-template int relist<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t, int);
+template int relist<float>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
+template int relist<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template int density_alpha<float>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template int density_alpha<double>(const TsphParams&, TsphObject*, TsphObject*, TsphControl*, void*, size_t, hipStream_t);
 template int solver_begin<float>(const TsphParams&, TsphControl*, int, int*, int, hipStream_t);
