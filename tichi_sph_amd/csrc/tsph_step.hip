@@ -104,7 +104,7 @@ __device__ __forceinline__ void pass_a_epilogue(const KP<T>& k, int i, const V4<
 }
 
 template <class T, bool FQ, bool XM>
-__device__ __forceinline__ void k_pass_a_fluid_body(Blk blk, KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
+__device__ __forceinline__ void k_pass_a_fluid_body(Blk blk, const KP<T>& k, int n, int cap, const V4<T>* __restrict__ fpos,
                    const T* __restrict__ fmass, const T* __restrict__ frvol,
                    const int* __restrict__ fstart, int nb, const V4<T>* __restrict__ bpos,
                    const T* __restrict__ bmass, const T* __restrict__ brvol,
@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(kBlock)
 // (when given): with `kept` they are read instead of summed, otherwise summed
 // and stored.
 template <class T, bool FQ>
-__device__ __forceinline__ void k_pass_a_bound_body(Blk blk, KP<T> k, int nb, const V4<T>* __restrict__ bpos, const T* __restrict__ bmass,
+__device__ __forceinline__ void k_pass_a_bound_body(Blk blk, const KP<T>& k, int nb, const V4<T>* __restrict__ bpos, const T* __restrict__ bmass,
                    const T* __restrict__ brvol, const int* __restrict__ bstart,
                    const V4<T>* __restrict__ fpos, const T* __restrict__ fmass,
                    const T* __restrict__ frvol, const int* __restrict__ fstart,
@@ -406,7 +406,7 @@ __global__ void k_solver_begin(KP<T> k, TsphControl* ctrl, int solve, int* signa
 // Jacobi half-step P: IPPE_adv_psi_init + IPPE_adv_psi (f,f)(f,b) +
 // IPPE_psi_adv_non_negative (sph.py:196-226) for iteration kk of the solve.
 template <class T>
-__device__ __forceinline__ void k_solver_psi_body(Blk blk, KP<T> k, int n, int cap, int kk, const V4<T>* __restrict__ fpos,
+__device__ __forceinline__ void k_solver_psi_body(Blk blk, const KP<T>& k, int n, int cap, int kk, const V4<T>* __restrict__ fpos,
                  const V4<T>* __restrict__ vadv, const T* __restrict__ psi_rho,
                  const T* __restrict__ rdens, const T* __restrict__ alpha,
                  const int* __restrict__ fstart, const V4<T>* __restrict__ bpos,
@@ -501,7 +501,7 @@ __global__ void __launch_bounds__(kBlock)
 // within the step), so the sweep only visits the fluid list, gathering the
 // one record (x_j, kappa_j) that the psi half-step wrote.
 template <class T>
-__device__ __forceinline__ void k_solver_update_body(Blk blk, KP<T> k, int n, int cap, int kk, const V4<T>* __restrict__ fpos,
+__device__ __forceinline__ void k_solver_update_body(Blk blk, const KP<T>& k, int n, int cap, int kk, const V4<T>* __restrict__ fpos,
                     const T* __restrict__ fmass, const T* __restrict__ kappa,
                     const V4<T>* __restrict__ krec, const V4<T>* __restrict__ alpha_1,
                     const int* __restrict__ fstart, const V4<T>* __restrict__ bpos,
@@ -597,7 +597,7 @@ __global__ void __launch_bounds__(kBlock)
 // the next kernel (k_surface_tension).
 // ---------------------------------------------------------------------------
 template <class T>
-__device__ __forceinline__ void k_advect_body(Blk blk, KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos, const T* __restrict__ vf,
+__device__ __forceinline__ void k_advect_body(Blk blk, const KP<T>& k, int n, int cap, const V4<T>* __restrict__ fpos, const T* __restrict__ vf,
              const T* __restrict__ rdens, const int* __restrict__ fstart,
              const int* __restrict__ bstart, const V4<T>* __restrict__ bpos,
              const int* __restrict__ nbr_f, const int* __restrict__ nbr_nf,
@@ -682,7 +682,7 @@ __global__ void __launch_bounds__(kBlock)
 // (its records are 44 B; the adjoint walk's 64-B records keep kStU).
 constexpr int kStFwdU = 8;
 template <class T>
-__device__ __forceinline__ void k_surface_tension_body(Blk blk, KP<T> k, int n, const V4<T>* __restrict__ fpos,
+__device__ __forceinline__ void k_surface_tension_body(Blk blk, const KP<T>& k, int n, const V4<T>* __restrict__ fpos,
                       const T* __restrict__ psi_rho,
                       const T* __restrict__ rdens, const T* __restrict__ vf,
                       const int* __restrict__ fstart, const V4<T>* __restrict__ normal,
@@ -989,7 +989,7 @@ __global__ void __launch_bounds__(kBlock)
 // SPH_update_energy (sph.py:265-286, 375-379).
 // ---------------------------------------------------------------------------
 template <class T>
-__device__ __forceinline__ void k_tail_body(Blk blk, KP<T> k, int n, int fbm, V4<T>* __restrict__ pos, V4<T>* __restrict__ vel,
+__device__ __forceinline__ void k_tail_body(Blk blk, const KP<T>& k, int n, int fbm, V4<T>* __restrict__ pos, V4<T>* __restrict__ vel,
            const V4<T>* __restrict__ vadv, T* __restrict__ mass, T* __restrict__ rdens,
            const T* __restrict__ rvol, T* __restrict__ vf, const T* __restrict__ tmp,
            const int* __restrict__ flag, double* __restrict__ partial, TsphControl* ctrl) {
