@@ -192,14 +192,19 @@ __host__ __device__ inline size_t batch_table_bytes(int ns) {
 __host__ __device__ inline long sort_len(long node_num) { return node_num + 1; }
 __host__ __device__ inline long cell_tiles(long node_num) { return scan_blocks(sort_len(node_num) + 1); }
 
-__device__ __forceinline__ int scene_of(const int* first, int ns, int t) {
-  int lo = 0, hi = ns - 1;  // largest s with first[s] <= t
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (first[mid] <= t) lo = mid;
-    else hi = mid - 1;
+// The largest scene s with first[s] <= t, for a whole wave (all 64 lanes
+// active, t wave-uniform): the lanes test 64 prefix entries at once and the
+// vote counts those <= t, so the lookup costs one load per 64 scenes instead
+// of a binary search's chain of dependent loads.  first[] is non-decreasing,
+// so the entries <= t form a prefix.
+__device__ __forceinline__ int scene_of_wave(const int* first, int ns, int t) {
+  const int lane = (int)(threadIdx.x & 63);
+  for (int c = 0; c < ns; c += 64) {
+    const int s = c + lane;
+    const int k = __popcll(__ballot(s < ns && first[s] <= t));
+    if (k < 64) return c + k - 1;
   }
-  return lo;
+  return ns - 1;
 }
 
 // Grid-stride loop over the tiles of one kind; f(entry, scene, blk).
@@ -213,7 +218,7 @@ __device__ __forceinline__ void for_tiles(const char* __restrict__ tab, int ns, 
   for (int t = blockIdx.x; t < total; t += gridDim.x) {
     // uniform per block: readfirstlane lets the scene's constants live in
     // scalar registers
-    const int s = __builtin_amdgcn_readfirstlane(scene_of(first, ns, t));
+    const int s = __builtin_amdgcn_readfirstlane(scene_of_wave(first, ns, t));
     const int f0 = first[s];
     f(E[s], s, Blk{t - f0, first[s + 1] - f0});
   }
