@@ -207,21 +207,32 @@ __device__ __forceinline__ int scene_of_wave(const int* first, int ns, int t) {
   return ns - 1;
 }
 
-// Grid-stride loop over the tiles of one kind; f(entry, scene, blk).
+// The tile prefix of one kind: first[s] is scene s's first tile, first[ns]
+// the kind's tile count.
+template <class T, int KIND>
+__device__ __forceinline__ const int* tile_prefix(const char* __restrict__ tab, int ns) {
+  return (const int*)(tab + batch_entries_bytes<T>(ns)) + KIND * (ns + 1);
+}
+
+// Runs f(entry, scene, blk) for tile t of one kind (t uniform per block).
 template <class T, int KIND, class F>
-__device__ __forceinline__ void for_tiles(const char* __restrict__ tab, int ns, F&& f) {
+__device__ __forceinline__ void on_tile(const char* __restrict__ tab, int ns, int t, F&& f) {
   // read-only here (the sort's pointer swap is a separate kernel), so the
   // scene entries can be scalar loads
   const SceneEntry<T>* __restrict__ E = (const SceneEntry<T>*)tab;
-  const int* first = (const int*)(tab + batch_entries_bytes<T>(ns)) + KIND * (ns + 1);
-  const int total = first[ns];
-  for (int t = blockIdx.x; t < total; t += gridDim.x) {
-    // uniform per block: readfirstlane lets the scene's constants live in
-    // scalar registers
-    const int s = __builtin_amdgcn_readfirstlane(scene_of_wave(first, ns, t));
-    const int f0 = first[s];
-    f(E[s], s, Blk{t - f0, first[s + 1] - f0});
-  }
+  const int* first = tile_prefix<T, KIND>(tab, ns);
+  // uniform per block: readfirstlane lets the scene's constants live in
+  // scalar registers
+  const int s = __builtin_amdgcn_readfirstlane(scene_of_wave(first, ns, t));
+  const int f0 = first[s];
+  f(E[s], s, Blk{t - f0, first[s + 1] - f0});
+}
+
+// Grid-stride loop over the tiles of one kind; f(entry, scene, blk).
+template <class T, int KIND, class F>
+__device__ __forceinline__ void for_tiles(const char* __restrict__ tab, int ns, F&& f) {
+  const int total = tile_prefix<T, KIND>(tab, ns)[ns];
+  for (int t = blockIdx.x; t < total; t += gridDim.x) on_tile<T, KIND>(tab, ns, t, f);
 }
 
 inline int grid_for(int tiles, int grid) {

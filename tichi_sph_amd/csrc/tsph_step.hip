@@ -1435,35 +1435,51 @@ __global__ void kb_mark_static_sums(char* tab, int ns) {
   if (b.static_sums != nullptr && b.n > 0) b.flags |= TSPH_OBJ_STATIC_SUMS;
 }
 
+// Pass A of one fluid tile / one boundary tile of a scene.
 template <class T>
-__global__ void __launch_bounds__(kBlock) kb_pass_a_fluid(const char* __restrict__ tab, int ns) {
-  for_tiles<T, kTileFluid>(tab, ns, [&](const SceneEntry<T>& e, int, Blk blk) {
-    const TsphObject& f = e.f;
-    const TsphObject& b = e.b;
-    pass_a_fluid_variant(e.k, [&](auto fq, auto xm) {
-      k_pass_a_fluid_body<T, decltype(fq)::value, decltype(xm)::value>(
-          blk, e.k, f.n, f.capacity, (const V4<T>*)f.pos, (const T*)f.mass,
-          (const T*)f.rest_volume, f.cell_start, b.n, (const V4<T>*)b.pos, (const T*)b.mass,
-          (const T*)b.rest_volume, b.cell_start, (T*)f.W, (T*)f.sph_compression,
-          (T*)f.sph_density, (V4<T>*)f.alpha_1, (T*)f.alpha_2, (T*)f.alpha, (T*)f.vol,
-          (const V4<T>*)f.vel, (V4<T>*)f.vel_adv, f.nbr_f, f.nbr_b, f.nbr_nf, f.nbr_nb, f.flag,
-          (V4<T>*)e.L.a1b, e.ctrl);
-    });
+__device__ __forceinline__ void pass_a_fluid_tile(const SceneEntry<T>& e, Blk blk) {
+  const TsphObject& f = e.f;
+  const TsphObject& b = e.b;
+  pass_a_fluid_variant(e.k, [&](auto fq, auto xm) {
+    k_pass_a_fluid_body<T, decltype(fq)::value, decltype(xm)::value>(
+        blk, e.k, f.n, f.capacity, (const V4<T>*)f.pos, (const T*)f.mass,
+        (const T*)f.rest_volume, f.cell_start, b.n, (const V4<T>*)b.pos, (const T*)b.mass,
+        (const T*)b.rest_volume, b.cell_start, (T*)f.W, (T*)f.sph_compression,
+        (T*)f.sph_density, (V4<T>*)f.alpha_1, (T*)f.alpha_2, (T*)f.alpha, (T*)f.vol,
+        (const V4<T>*)f.vel, (V4<T>*)f.vel_adv, f.nbr_f, f.nbr_b, f.nbr_nf, f.nbr_nb, f.flag,
+        (V4<T>*)e.L.a1b, e.ctrl);
   });
 }
-
 template <class T>
-__global__ void __launch_bounds__(kBlock) kb_pass_a_bound(const char* __restrict__ tab, int ns) {
-  for_tiles<T, kTileBound>(tab, ns, [&](const SceneEntry<T>& e, int, Blk blk) {
-    const TsphObject& f = e.f;
-    const TsphObject& b = e.b;
-    k_pass_a_bound_body<T, false>(
-        blk, e.k, b.n, (const V4<T>*)b.pos, (const T*)b.mass, (const T*)b.rest_volume,
-        b.cell_start, (const V4<T>*)f.pos, (const T*)f.mass, (const T*)f.rest_volume,
-        f.cell_start, (T*)b.W, (T*)b.sph_compression, (T*)b.sph_density, (V4<T>*)b.alpha_1,
-        (T*)b.alpha_2, (T*)b.alpha, (T*)b.vol, (V4<T>*)b.static_sums,
-        (b.flags & TSPH_OBJ_STATIC_SUMS) != 0);
-  });
+__device__ __forceinline__ void pass_a_bound_tile(const SceneEntry<T>& e, Blk blk) {
+  const TsphObject& f = e.f;
+  const TsphObject& b = e.b;
+  k_pass_a_bound_body<T, false>(
+      blk, e.k, b.n, (const V4<T>*)b.pos, (const T*)b.mass, (const T*)b.rest_volume,
+      b.cell_start, (const V4<T>*)f.pos, (const T*)f.mass, (const T*)f.rest_volume,
+      f.cell_start, (T*)b.W, (T*)b.sph_compression, (T*)b.sph_density, (V4<T>*)b.alpha_1,
+      (T*)b.alpha_2, (T*)b.alpha, (T*)b.vol, (V4<T>*)b.static_sums,
+      (b.flags & TSPH_OBJ_STATIC_SUMS) != 0);
+}
+
+// Pass A of every scene's fluid and boundary in one grouped launch: tiles
+// [0, fluid tiles) are fluid tiles, the rest boundary tiles.  The two
+// halves write only their own object's fields, so the boundary tiles fill
+// the tail of the fluid tiles' launch instead of waiting for it.
+template <class T>
+__global__ void __launch_bounds__(kBlock) kb_pass_a_both(const char* __restrict__ tab, int ns) {
+  const int tf = tile_prefix<T, kTileFluid>(tab, ns)[ns];
+  const int total = tf + tile_prefix<T, kTileBound>(tab, ns)[ns];
+  for (int t = blockIdx.x; t < total; t += gridDim.x) {
+    if (t < tf)
+      on_tile<T, kTileFluid>(tab, ns, t, [&](const SceneEntry<T>& e, int, Blk blk) {
+        pass_a_fluid_tile(e, blk);
+      });
+    else
+      on_tile<T, kTileBound>(tab, ns, t - tf, [&](const SceneEntry<T>& e, int, Blk blk) {
+        pass_a_bound_tile(e, blk);
+      });
+  }
 }
 
 template <class T>
@@ -1555,12 +1571,10 @@ template <class T>
 int batch_density_alpha(const TsphBatchInfo& bi, char* tab, int grid, hipStream_t st) {
   const int ns = bi.nscenes;
   hipLaunchKernelGGL(kb_step_reset<T>, dim3(ns), dim3(kBlock), 0, st, tab, ns);
-  if (int g = grid_for(bi.tiles[kTileFluid], grid))
-    hipLaunchKernelGGL(kb_pass_a_fluid<T>, dim3(g), dim3(kBlock), 0, st, tab, ns);
-  if (int g = grid_for(bi.tiles[kTileBound], grid)) {
-    hipLaunchKernelGGL(kb_pass_a_bound<T>, dim3(g), dim3(kBlock), 0, st, tab, ns);
+  if (int g = grid_for(bi.tiles[kTileFluid] + bi.tiles[kTileBound], grid))
+    hipLaunchKernelGGL(kb_pass_a_both<T>, dim3(g), dim3(kBlock), 0, st, tab, ns);
+  if (bi.tiles[kTileBound] > 0)
     hipLaunchKernelGGL(kb_mark_static_sums<T>, dim3((ns + 63) / 64), dim3(64), 0, st, tab, ns);
-  }
   TSPH_LAUNCH_CHECK("batch_density_alpha");
   return 0;
 }
