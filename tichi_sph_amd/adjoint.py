@@ -240,17 +240,17 @@ class DiffRunner:
         P = ctypes.byref(sim.params)
         ws, wsb = sim._ws()
         stp = sim.stream_ptr()
-        f.t["pos"][:n].copy_(tp.pos)
-        f.t["mass"][:n].copy_(tp.mass)
-        f.t["rest_density"][:n].copy_(tp.rest_density)
-        f.t["cell_start"].copy_(tp.cell_start)
-        f.t["sph_density"][:n].copy_(tp.density)
-        f.t["alpha"][:n].copy_(tp.alpha)
-        f.t["vol"][:n].copy_(tp.vol)
-        f.t["alpha_1"][:n].copy_(tp.alpha_1)
-        if tp.nbr_f is not None:
-            self._use_lists(tp.nbr_f, tp.nbr_nf, tp.nbr_nb)
-        fs, bs = ctypes.byref(f.struct()), ctypes.byref(b.struct())
+        # the reverse kernels read the recorded fields (and lists) through a
+        # copy of the fluid's descriptor that points at the tape: nothing is
+        # copied back, and no reverse kernel writes these fields
+        fsd = _lib.TsphObject.from_buffer_copy(f.struct())
+        for name, t in (("pos", tp.pos), ("mass", tp.mass), ("rest_density", tp.rest_density),
+                        ("cell_start", tp.cell_start), ("sph_density", tp.density),
+                        ("alpha", tp.alpha), ("vol", tp.vol), ("alpha_1", tp.alpha_1),
+                        ("nbr_f", tp.nbr_f), ("nbr_nf", tp.nbr_nf), ("nbr_nb", tp.nbr_nb)):
+            if t is not None:
+                setattr(fsd, name, t.data_ptr())
+        fs, bs = ctypes.byref(fsd), ctypes.byref(b.struct())
         A = ctypes.byref(self.adj)
         vp = ctypes.c_void_p
         def last_kappa(pairs):
@@ -285,7 +285,6 @@ class DiffRunner:
         sim._call("adj_density_alpha", self.lib.tsph_adj_density_alpha, P, fs, bs, A, stp)
         sim._call("adj_end", self.lib.tsph_adj_end, P, fs, A, vp(tp.pid.data_ptr()),
                   vp(xbar_ref.data_ptr()), vp(vbar_ref.data_ptr()), stp)
-        self._use_lists(*self._own_lists)
 
     def _use_lists(self, nbr_f, nbr_nf, nbr_nb):
         """Points the fluid's list buffers (tensors and descriptor) at these:
