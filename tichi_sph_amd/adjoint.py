@@ -27,11 +27,12 @@ alpha_1 of pass A, vel_adv and kappa of every Jacobi iteration, and the
 advection's velocity and normals, and (`DiffRunner.tape_lists`, on by
 default) the fluid neighbour lists: a taped step's pass A writes them into
 a list buffer of the tape's own (list_cap_f rows of the capacity, 4 bytes
-each, with both lists' counts), so they cost no copy; without stored lists `tsph_relist`
-recomputes them bit-identically from the recorded positions.  The boundary
-enters the reverse only through per-particle sums over the boundary
-neighbours (`tsph_adj_bound_sums`, recomputed for each reversed step), so
-the boundary lists are neither stored nor rebuilt.
+each, with both lists' counts), so they cost no copy; without stored lists
+`tsph_relist` recomputes them bit-identically from the recorded positions.
+The boundary enters the reverse only through per-particle sums over the
+boundary neighbours (`tsph_adj_bound_sums`, recomputed for each reversed
+step), so the boundary lists are neither stored nor rebuilt.  The reverse
+reads a tape in place, through a copy of the fluid's descriptor.
 
     runner = DiffRunner(sim)
     x, v = rollout(runner, v0, mu, gamma, steps=200, every=20)
@@ -220,7 +221,9 @@ class DiffRunner:
         i_it = sim._solve(1, tape=tp.iters[1] if tape else None, force=force[1], frozen=fz[1])
         sim._call("tail", sim.lib.tsph_tail, P, fs, sim.ctrl_ptr(), 0, ws, wsb, stp)
         if tape:
-            tp.calls = sim.calls  # no later call: lists and density / alpha still resident
+            # while no later call runs, the Simulation's lists are this step's
+            # (what a tape without stored lists relies on)
+            tp.calls = sim.calls
         # the Simulation keeps its own list buffers between steps, so that no
         # later step writes into a tape's
         self._use_lists(*self._own_lists)
@@ -236,7 +239,6 @@ class DiffRunner:
         is consumed (its vel_adv records carry psi_bar afterwards)."""
         sim = self.sim
         f, b = sim.fluid, sim.bound
-        n = f.part_num
         P = ctypes.byref(sim.params)
         ws, wsb = sim._ws()
         stp = sim.stream_ptr()

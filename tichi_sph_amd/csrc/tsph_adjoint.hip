@@ -235,12 +235,12 @@ __global__ void __launch_bounds__(kBlock)
   BoundSums<T> bs{};
   if (hb) bs = load_bound_sums(bmom, cap, i);
   T kb = 0, ax = 0, ay = 0, az = 0;
-  auto term = [&](const Rec2<T>& d, bool fluid) {
+  auto term = [&](const Rec2<T>& d) {  // a fluid neighbour j
     const V4<T>& q = d.a;   // q.w = X_j = m_j
-    const V4<T>& uj = d.b;  // xyz = u_j, w = kappa_j (zero for the boundary)
+    const V4<T>& uj = d.b;  // xyz = u_j, w = kappa_j
     T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     D2<T> e = kern_d2(k, dx * dx + dy * dy + dz * dz);
-    T rmj = fluid ? T(1) / q.w : T(0);
+    T rmj = T(1) / q.w;
     T mx = wx - uj.x * rmj, my = wy - uj.y * rmj, mz = wz - uj.z * rmj;
     kb += q.w * e.g * (dx * mx + dy * my + dz * mz);
     T c = ki * q.w + uj.w * Xi;
@@ -251,7 +251,7 @@ __global__ void __launch_bounds__(kBlock)
   sweep_lists_coop<kCoop, T, Rec2<T>>(
       k, cap, i, cl, p, fstart, nullptr, fpos, nullptr, nbr_f, nullptr, nbr_nf[i], 0,
       [&](int j) { return Rec2<T>{fpos[j], vbar[j]}; }, [&](int) { return Rec2<T>{}; },
-      [&](const Rec2<T>& d) { term(d, true); }, [&](const Rec2<T>&) {});
+      term, [&](const Rec2<T>&) {});
   kb = group_sum<kCoop>(kb);
   ax = group_sum<kCoop>(ax);
   ay = group_sum<kCoop>(ay);
