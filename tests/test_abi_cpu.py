@@ -39,7 +39,8 @@ def test_struct_layout_matches_header(tmp_path):
                        "static_sums"],
         "TsphControl": ["dt", "it", "stop", "general_flag", "ke", "run_u", "err_div", "err_incom", "signal", "token",
                         "overflow_f", "counter2"],
-        "TsphAdjoint": ["xbar", "vbar_out", "nbar", "a1bar", "kappa_next", "tape_normal", "grad"],
+        "TsphAdjoint": ["xbar", "vbar_out", "nbar", "a1bar", "kappa_next", "tape_normal", "grad", "bmom",
+                        "bflag"],
     }
     src = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HDR}"', "int main(void){"]
     for s, fl in fields.items():
@@ -107,3 +108,23 @@ def test_batch_table_sizing_and_argument_checks():
     rc = lib.tsph_batch_density_alpha(ctypes.byref(info), None, 0, None)
     assert rc == -1
 
+
+
+def test_adjoint_boundary_sums_argument_checks():
+    """The entries that read or write the boundary sums refuse an adjoint
+    without their buffers before any device work (host-only scene)."""
+    import ctypes
+    from conftest import load_scene
+    from tichi_sph_amd import _lib, simulation_from_dicts
+    lib = _lib.load()
+    cb, sb = load_scene("dambreak_3d_8k")
+    sim = simulation_from_dicts(cb, sb, device="cpu", cfl=False)
+    P = ctypes.byref(sim.params)
+    fs, bs = ctypes.byref(sim.fluid.struct()), ctypes.byref(sim.bound.struct())
+    adj = _lib.TsphAdjoint()  # every pointer NULL
+    for name, args in (("tsph_adj_bound_sums", (P, fs, bs, ctypes.byref(adj), None)),
+                       ("tsph_adj_density_alpha", (P, fs, bs, ctypes.byref(adj), None)),
+                       ("tsph_adj_solver_iteration", (P, fs, bs, None, ctypes.byref(adj), None, 0, None))):
+        rc = getattr(lib, name)(*args)
+        assert rc == -1, name  # TSPH_EINVAL
+        assert b"null boundary sums" in lib.tsph_last_error(), name
