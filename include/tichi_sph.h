@@ -424,7 +424,8 @@ int tsph_adj_begin(const TsphParams* p, TsphObject* fluid, TsphControl* ctrl, Ts
 /* Reverse of one Jacobi iteration (IPPE_adv_psi + IPPE_psi_adv_non_negative
  * + IPPE_update_vel_adv, sph.py:196-244) whose record is adj->tape_vel_adv,
  * with that iteration's kappa in vbar's w lane: vbar -> vbar_out,
- * accumulating xbar, rhobar, alphabar. */
+ * accumulating xbar, rhobar, alphabar.  The (f,b) pairs come from the step's
+ * boundary sums (tsph_adj_bound_sums); `bound` is not read. */
 int tsph_adj_solver_iteration(const TsphParams* p, TsphObject* fluid, TsphObject* bound,
                               TsphControl* ctrl, TsphAdjoint* adj, void* workspace,
                               size_t ws_bytes, void* stream);
@@ -438,7 +439,8 @@ int tsph_adj_advect(const TsphParams* p, TsphObject* fluid, TsphControl* ctrl, T
                     void* workspace, size_t ws_bytes, void* stream);
 
 /* Reverse of density and alpha (SPH_prepare_attr, _alpha_1, _alpha_2,
- * _alpha; sph.py:50-105): accumulates xbar from rhobar and alphabar. */
+ * _alpha; sph.py:50-105): accumulates xbar from rhobar and alphabar.  The
+ * (f,b) pairs come from the step's boundary sums (tsph_adj_bound_sums). */
 int tsph_adj_density_alpha(const TsphParams* p, TsphObject* fluid, TsphObject* bound,
                            TsphAdjoint* adj, void* stream);
 
