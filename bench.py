@@ -347,7 +347,7 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (--gpus must be 1)",
+        print("bench.py measures one GPU: tichi_sph_amd has no multi-GPU path (--gpus must be 1)",
               file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
