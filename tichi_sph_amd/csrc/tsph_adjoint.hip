@@ -240,7 +240,7 @@ __global__ void __launch_bounds__(kBlock)
     const V4<T>& uj = d.b;  // xyz = u_j, w = kappa_j
     T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     D2<T> e = kern_d2(k, dx * dx + dy * dy + dz * dz);
-    T rmj = T(1) / q.w;
+    T rmj = f_rcp(q.w);  // hardware reciprocal in FP32 (the reverse is not a bit-exact sum)
     T mx = wx - uj.x * rmj, my = wy - uj.y * rmj, mz = wz - uj.z * rmj;
     kb += q.w * e.g * (dx * mx + dy * my + dz * mz);
     T c = ki * q.w + uj.w * Xi;
@@ -495,7 +495,7 @@ __global__ void __launch_bounds__(kBlock)
       T r2 = dx * dx + dy * dy + dz * dz;
       D2<T> e = kern_d2(k, r2);
       T den = k.eps + r2;
-      T rden = T(1) / den;
+      T rden = f_rcp(den);
       T w = e.g * rden;
       T wq = (e.hq - T(2) * w) * rden;  // w'(r) / r
       T Vj = vj.w, fvj = aj.w;
@@ -527,7 +527,7 @@ __global__ void __launch_bounds__(kBlock)
         [&](int j) {
           const V4<T> v = tvel[j];  // w = V_j
           V4<T> u = vbar[j];
-          V4<T> a = v4(dt * u.x, dt * u.y, dt * u.z, k.two_d2 * k.mu / rdens[j]);
+          V4<T> a = v4(dt * u.x, dt * u.y, dt * u.z, k.two_d2 * k.mu * f_rcp(rdens[j]));
           return Rec4<T>{fpos[j], v, a, nbar ? nbar[j] : V4<T>{}};
         },
         [&](int j) { return Rec4<T>{}; }, term, [&](const Rec4<T>&) {});
@@ -622,7 +622,7 @@ __global__ void __launch_bounds__(kBlock)
         T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
         D2<T> e = kern_d2(k, dx * dx + dy * dy + dz * dz);
         T cd = (rbi * mj + rbj * mi) * e.g;
-        T c2 = T(2) * e.g * e.w2 * (abi.w * q.w * q.w / mj + abj.w * xm_i);
+        T c2 = T(2) * e.g * e.w2 * (abi.w * q.w * q.w * f_rcp(mj) + abj.w * xm_i);
         T s1 = cd + c2;
         ax += s1 * dx;
         ay += s1 * dy;

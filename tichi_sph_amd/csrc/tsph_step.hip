@@ -622,7 +622,7 @@ __device__ __forceinline__ void k_advect_body(Blk blk, const KP<T>& k, int n, in
       T r2 = dx * dx + dy * dy + dz * dz, r;
       T g = f_grad(k, r2, &r);
       T Ax = (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz;
-      T s = fvis * Vj * g * Ax / (k.eps + r2);  // W_lap, sph_util.py:40-43
+      T s = fvis * Vj * g * Ax * f_rcp(k.eps + r2);  // W_lap, sph_util.py:40-43 (fast quotient)
       ax += s * dx;
       ay += s * dy;
       az += s * dz;
