@@ -98,6 +98,11 @@ __device__ __forceinline__ void add_J(T& ax, T& ay, T& az, const D2<T>& d, T dx,
 // so the reverse sweeps read three records instead of walking the boundary.
 // ---------------------------------------------------------------------------
 constexpr int kBsumU = 8;  // boundary candidates in flight per lane
+// Lanes per particle of the reverse Jacobi and viscosity sweeps.  Their pair
+// terms (kernel second derivatives, Hessian products) make them VALU-bound
+// about as much as gather-bound, so two lanes per particle, which leave fewer
+// lanes idle at the end of uneven lists, beat the forward sweeps' four.
+constexpr int kCoopR = 2;
 
 template <class T>
 __global__ void __launch_bounds__(kBlock)
@@ -214,11 +219,11 @@ __global__ void __launch_bounds__(kBlock)
                 const int* __restrict__ bflag, V4<T>* __restrict__ tvel, T* __restrict__ rhobar,
                 T* __restrict__ alphabar, V4<T>* __restrict__ xbar, const TsphControl* ctrl) {
   const int base = xcd_block(blockIdx.x, gridDim.x) * kBlock;
-  const int cl = coop_lane<kCoop>();
+  const int cl = coop_lane<kCoopR>();
   const T dt = (T)ctrl->dt;
 #pragma unroll 1
-  for (int round = 0; round < kCoop; ++round) {
-  const int i = coop_particle<kCoop>(base, round);
+  for (int round = 0; round < kCoopR; ++round) {
+  const int i = coop_particle<kCoopR>(base, round);
   if (i >= n) break;
   const V4<T> p = fpos[i];
   const T Xi = p.w;
@@ -248,14 +253,14 @@ __global__ void __launch_bounds__(kBlock)
   };
   // (f,f) pairs; the (f,b) pairs (u_b = kappa_b = 0) are w_l . A_l and
   // kappa_l M_l w_l from the boundary sums
-  sweep_lists_coop<kCoop, T, Rec2<T>>(
+  sweep_lists_coop<kCoopR, T, Rec2<T>>(
       k, cap, i, cl, p, fstart, nullptr, fpos, nullptr, nbr_f, nullptr, nbr_nf[i], 0,
       [&](int j) { return Rec2<T>{fpos[j], vbar[j]}; }, [&](int) { return Rec2<T>{}; },
       term, [&](const Rec2<T>&) {});
-  kb = group_sum<kCoop>(kb);
-  ax = group_sum<kCoop>(ax);
-  ay = group_sum<kCoop>(ay);
-  az = group_sum<kCoop>(az);
+  kb = group_sum<kCoopR>(kb);
+  ax = group_sum<kCoopR>(ax);
+  ay = group_sum<kCoopR>(ay);
+  az = group_sum<kCoopR>(az);
   if (cl != 0) continue;
   if (hb) {
     kb += bs.dot_A(wx, wy, wz);
@@ -287,11 +292,11 @@ __global__ void __launch_bounds__(kBlock)
               const V4<T>* __restrict__ bmom, const int* __restrict__ bflag,
               V4<T>* __restrict__ vbar_out, V4<T>* __restrict__ xbar, const TsphControl* ctrl) {
   const int base = xcd_block(blockIdx.x, gridDim.x) * kBlock;
-  const int cl = coop_lane<kCoop>();
+  const int cl = coop_lane<kCoopR>();
   const T dt = (T)ctrl->dt;
 #pragma unroll 1
-  for (int round = 0; round < kCoop; ++round) {
-  const int i = coop_particle<kCoop>(base, round);
+  for (int round = 0; round < kCoopR; ++round) {
+  const int i = coop_particle<kCoopR>(base, round);
   if (i >= n) break;
   const V4<T> p = fpos[i];
   const T Xi = p.w;
@@ -320,16 +325,16 @@ __global__ void __launch_bounds__(kBlock)
   };
   // (f,f) pairs; the (f,b) pairs (v_b = psib_b = 0) are psib_l A_l and
   // psib_l M_l v_l from the boundary sums
-  sweep_lists_coop<kCoop, T, Rec2<T>>(
+  sweep_lists_coop<kCoopR, T, Rec2<T>>(
       k, cap, i, cl, p, fstart, nullptr, fpos, nullptr, nbr_f, nullptr, nbr_nf[i], 0,
       [&](int j) { return Rec2<T>{fpos[j], tvel[j]}; }, [&](int) { return Rec2<T>{}; }, term,
       [&](const Rec2<T>&) {});
-  vx = group_sum<kCoop>(vx);
-  vy = group_sum<kCoop>(vy);
-  vz = group_sum<kCoop>(vz);
-  ax = group_sum<kCoop>(ax);
-  ay = group_sum<kCoop>(ay);
-  az = group_sum<kCoop>(az);
+  vx = group_sum<kCoopR>(vx);
+  vy = group_sum<kCoopR>(vy);
+  vz = group_sum<kCoopR>(vz);
+  ax = group_sum<kCoopR>(ax);
+  ay = group_sum<kCoopR>(ay);
+  az = group_sum<kCoopR>(az);
   if (cl != 0) continue;
   if (hb) {
     vx += pbi * bs.m0.x;
@@ -465,11 +470,11 @@ __global__ void __launch_bounds__(kBlock)
                     const TsphControl* ctrl) {
   __shared__ double lds[kWaves];
   const int base = xcd_block(blockIdx.x, gridDim.x) * kBlock;
-  const int cl = coop_lane<kCoop>();
+  const int cl = coop_lane<kCoopR>();
   double msum = 0;
 #pragma unroll 1
-  for (int round = 0; round < kCoop; ++round) {
-    const int i = coop_particle<kCoop>(base, round);
+  for (int round = 0; round < kCoopR; ++round) {
+    const int i = coop_particle<kCoopR>(base, round);
     if (i >= n) break;
     const T dt = (T)ctrl->dt;
     const V4<T> p = fpos[i];
@@ -522,7 +527,7 @@ __global__ void __launch_bounds__(kBlock)
       xby += -k.h * (e.g * cy + s * dy);
       xbz += -k.h * (e.g * cz + s * dz);
     };
-    sweep_lists_coop<kCoop, T, Rec4<T>>(
+    sweep_lists_coop<kCoopR, T, Rec4<T>>(
         k, cap, i, cl, p, fstart, bstart, fpos, bpos, nbr_f, nullptr, nbr_nf[i], 0,
         [&](int j) {
           const V4<T> v = tvel[j];  // w = V_j
@@ -531,14 +536,14 @@ __global__ void __launch_bounds__(kBlock)
           return Rec4<T>{fpos[j], v, a, nbar ? nbar[j] : V4<T>{}};
         },
         [&](int j) { return Rec4<T>{}; }, term, [&](const Rec4<T>&) {});
-    vbx = group_sum<kCoop>(vbx);
-    vby = group_sum<kCoop>(vby);
-    vbz = group_sum<kCoop>(vbz);
-    xbx = group_sum<kCoop>(xbx);
-    xby = group_sum<kCoop>(xby);
-    xbz = group_sum<kCoop>(xbz);
-    Vb = group_sum<kCoop>(Vb);
-    ms = group_sum<kCoop>(ms);
+    vbx = group_sum<kCoopR>(vbx);
+    vby = group_sum<kCoopR>(vby);
+    vbz = group_sum<kCoopR>(vbz);
+    xbx = group_sum<kCoopR>(xbx);
+    xby = group_sum<kCoopR>(xby);
+    xbz = group_sum<kCoopR>(xbz);
+    Vb = group_sum<kCoopR>(Vb);
+    ms = group_sum<kCoopR>(ms);
     if (cl != 0) continue;
     vbar_out[i] = v4(ui.x + vbx, ui.y + vby, ui.z + vbz, kn);
     rhobar[i] = rb0 + -Vb * Vi / di;
