@@ -103,6 +103,11 @@ constexpr int kBsumU = 8;  // boundary candidates in flight per lane
 // about as much as gather-bound, so two lanes per particle, which leave fewer
 // lanes idle at the end of uneven lists, beat the forward sweeps' four.
 constexpr int kCoopR = 2;
+// Gathers in flight per lane in the reverse viscosity sweep.  Its four-vector
+// records take 152 VGPRs at four in flight (3 waves per SIMD); two keep it at
+// 110 (4 waves) and run faster.  The Jacobi sweeps keep four: at two they
+// reach 5 waves but lose more to latency than they gain.
+constexpr int kNbVis = 2;
 
 template <class T>
 __global__ void __launch_bounds__(kBlock)
@@ -527,7 +532,7 @@ __global__ void __launch_bounds__(kBlock)
       xby += -k.h * (e.g * cy + s * dy);
       xbz += -k.h * (e.g * cz + s * dz);
     };
-    sweep_lists_coop<kCoopR, T, Rec4<T>>(
+    sweep_lists_coop<kCoopR, T, Rec4<T>, kNbVis>(
         k, cap, i, cl, p, fstart, bstart, fpos, bpos, nbr_f, nullptr, nbr_nf[i], 0,
         [&](int j) {
           const V4<T> v = tvel[j];  // w = V_j

@@ -215,39 +215,41 @@ __device__ __forceinline__ T group_sum(T v) {
   return v;
 }
 
-// One list, lane c's share, software-pipelined like sweep_one_list.
-template <int C, class Rec, class Load, class Pair>
+// One list, lane c's share, software-pipelined like sweep_one_list, NB
+// gathers in flight per lane.
+template <int C, class Rec, int NB = kCoopNb, class Load, class Pair>
 __device__ __forceinline__ void sweep_one_list_coop(int cap, int i, int c,
                                                     const int* __restrict__ list, int n,
                                                     Load&& load, Pair&& pair) {
-  int j[kCoopNb];
+  int j[NB];
 #pragma unroll
-  for (int u = 0; u < kCoopNb; ++u) {
+  for (int u = 0; u < NB; ++u) {
     const int s = c + C * u;
     j[u] = s < n ? list[(long)s * cap + i] : -1;
   }
-  for (int s0 = c; s0 < n; s0 += C * kCoopNb) {
-    Rec d[kCoopNb];
+  for (int s0 = c; s0 < n; s0 += C * NB) {
+    Rec d[NB];
 #pragma unroll
-    for (int u = 0; u < kCoopNb; ++u)
+    for (int u = 0; u < NB; ++u)
       if (j[u] >= 0) d[u] = load(j[u]);
-    int jn[kCoopNb];
+    int jn[NB];
 #pragma unroll
-    for (int u = 0; u < kCoopNb; ++u) {
-      const int s = s0 + C * (kCoopNb + u);
+    for (int u = 0; u < NB; ++u) {
+      const int s = s0 + C * (NB + u);
       jn[u] = s < n ? list[(long)s * cap + i] : -1;
     }
 #pragma unroll
-    for (int u = 0; u < kCoopNb; ++u)
+    for (int u = 0; u < NB; ++u)
       if (j[u] >= 0) pair(d[u]);
 #pragma unroll
-    for (int u = 0; u < kCoopNb; ++u) j[u] = jn[u];
+    for (int u = 0; u < NB; ++u) j[u] = jn[u];
   }
 }
 
 // sweep_lists for lane c of a particle's group.  A particle whose list
 // overflowed is swept by lane 0 alone, walking the cells.
-template <int C, class T, class Rec, class LoadF, class LoadB, class PairF, class PairB>
+template <int C, class T, class Rec, int NB = kCoopNb, class LoadF, class LoadB, class PairF,
+          class PairB>
 __device__ __forceinline__ void sweep_lists_coop(const KP<T>& k, int cap, int i, int c, V4<T> p,
                                                  const int* __restrict__ fstart,
                                                  const int* __restrict__ bstart,
@@ -258,8 +260,8 @@ __device__ __forceinline__ void sweep_lists_coop(const KP<T>& k, int cap, int i,
                                                  LoadF&& load_f, LoadB&& load_b, PairF&& pair_f,
                                                  PairB&& pair_b) {
   if (nf <= k.list_cap_f && nbb <= k.list_cap_b) {
-    sweep_one_list_coop<C, Rec>(cap, i, c, nbr_f, nf, load_f, pair_f);
-    if (nbb > 0) sweep_one_list_coop<C, Rec>(cap, i, c, nbr_b, nbb, load_b, pair_b);
+    sweep_one_list_coop<C, Rec, NB>(cap, i, c, nbr_f, nf, load_f, pair_f);
+    if (nbb > 0) sweep_one_list_coop<C, Rec, NB>(cap, i, c, nbr_b, nbb, load_b, pair_b);
     return;
   }
   if (c != 0) return;
