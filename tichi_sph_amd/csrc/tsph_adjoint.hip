@@ -358,19 +358,25 @@ __global__ void __launch_bounds__(kBlock)
 //   f_lj = 2 rho0_l / (rho_l + rho_j),
 // over every candidate of the neighbouring cells with r > 0 (the curvature
 // term carries no kernel weight) and both volume fractions > 0.99.
-// With a_bar = dt u (vel_adv = vel + dt a, sph.py:247-250):
-//   n_bar_l  = gamma sum_j (f_lj a_l - f_jl a_j)
+// With a_bar = dt u (vel_adv = vel + dt a, sph.py:247-250) and
+// e_lj = f_lj a_l - f_jl a_j:
+//   n_bar_l  = gamma sum_j e_lj
 //   x_bar_l += -gamma sum_j Jc (f_lj m_j a_l - f_jl m_l a_j),
 //              Jc w = (C/r) w + ((C' - C/r)/r^2)(x.w) x
 //   rho_bar_l += -sum_j (f_lj s_lj + f_jl s_jl) / (rho_l + rho_j)
 //   d loss / d gamma += a_l . sum_j f_lj (-m_j C x/r + n_l - n_j)
+// The terms without x (the curvature's) are summed over the cells, gathering
+// no position: r > 0 is j != l except for a particle at l's own position,
+// which lies in l's own cell, so only that cell's candidates are tested.
+// The cohesion terms (C = C' = 0 beyond h) are summed over the in-range list.
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ void __launch_bounds__(kBlock)
-    k_adj_surface_tension(KP<T> k, int n, const V4<T>* __restrict__ fpos,
+    k_adj_surface_tension(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
                           const T* __restrict__ fmass, const T* __restrict__ dens,
                           const T* __restrict__ rdens, const T* __restrict__ vf,
-                          const int* __restrict__ fstart, const V4<T>* __restrict__ tnormal,
+                          const int* __restrict__ fstart, const int* __restrict__ nbr_f,
+                          const int* __restrict__ nbr_nf, const V4<T>* __restrict__ tnormal,
                           const V4<T>* __restrict__ vbar, V4<T>* __restrict__ nbar,
                           T* __restrict__ rhobar, V4<T>* __restrict__ xbar,
                           double* __restrict__ partial, const TsphControl* ctrl) {
@@ -389,52 +395,77 @@ __global__ void __launch_bounds__(kBlock)
       const T rhoi = dens[i], r0i = rdens[i], mi = fmass[i];
       int c[3];
       node_encode(k, p.x, p.y, p.z, c);
-      // per candidate: (x_j, X_j = m_j), the recorded normal with +-rho_j in
-      // its spare lane (negative where j's 0.99 gate fails; the forward
-      // advection writes it), vbar_j and rho0_j
-      struct Cand {
+      const int own = dim_encode(k, c[0], c[1], c[2]);
+      const int own0 = fstart[own], own1 = fstart[own + 1];
+      // curvature, per candidate: the recorded normal with +-rho_j in its
+      // spare lane (negative where j's 0.99 gate fails; the forward advection
+      // writes it), vbar_j and rho0_j
+      struct CandN {
+        V4<T> nj, uj;
+        T r0j;
+      };
+      auto loadn = [&](int j) { return CandN{tnormal[j], vbar[j], rdens[j]}; };
+      walk_cells_prefetch<kStU>(k, fstart, c, loadn, [&](int j, const CandN& d) {
+        if (j == i || !(d.nj.w > T(0))) return;
+        if (j >= own0 && j < own1) {
+          const V4<T> q = fpos[j];
+          if (!(s_r2(p.x - q.x, p.y - q.y, p.z - q.z) > T(0))) return;
+        }
+        const V4<T>& nj = d.nj;
+        const T rden = f_rcp(rhoi + nj.w);
+        const T fij = T(2) * r0i * rden, fji = T(2) * d.r0j * rden;
+        const T dnx = ni.x - nj.x, dny = ni.y - nj.y, dnz = ni.z - nj.z;
+        const T ex = fij * ali_x - fji * (dt * d.uj.x);
+        const T ey = fij * ali_y - fji * (dt * d.uj.y);
+        const T ez = fij * ali_z - fji * (dt * d.uj.z);
+        gs += fij * (ali_x * dnx + ali_y * dny + ali_z * dnz);
+        nbx += gam * ex;
+        nby += gam * ey;
+        nbz += gam * ez;
+        rb += -gam * rden * (dnx * ex + dny * ey + dnz * ez);
+      });
+      // cohesion, per in-range neighbour: (x_j, X_j = m_j) as well
+      struct CandC {
         V4<T> q, nj, uj;
         T r0j;
       };
-      auto load = [&](int j) { return Cand{fpos[j], tnormal[j], vbar[j], rdens[j]}; };
-      walk_cells_prefetch<kStU>(k, fstart, c, load, [&](int, const Cand& d) {
+      auto loadc = [&](int j) { return CandC{fpos[j], tnormal[j], vbar[j], rdens[j]}; };
+      auto coh = [&](const CandC& d) {
+        if (!(d.nj.w > T(0))) return;
         const V4<T>& q = d.q;
-        T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-        T r2 = dx * dx + dy * dy + dz * dz;
-        if (!(r2 > T(0)) || !(d.nj.w > T(0))) return;
+        const T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+        const T r2 = dx * dx + dy * dy + dz * dz;
         const T rinv = f_rsqrt(r2);
-        T r = r2 * rinv;
         T C, Cp;
-        coh_d1(k, r, &C, &Cp);
-        T cr = C * rinv;
-        T cq = (Cp - cr) * (rinv * rinv);
-        const V4<T>& nj = d.nj;
-        const V4<T>& uj = d.uj;
-        T alj_x = dt * uj.x, alj_y = dt * uj.y, alj_z = dt * uj.z;
-        T rhoj = d.nj.w, r0j = d.r0j, mj = q.w;  // DFSPH: the alias X is the mass
-        T den = rhoi + rhoj;
-        const T rden = f_rcp(den);
-        T fij = T(2) * r0i * rden, fji = T(2) * r0j * rden;
-        T dnx = ni.x - nj.x, dny = ni.y - nj.y, dnz = ni.z - nj.z;
-        // gamma-free force of the pair on l
-        T tx = -mj * cr * dx + dnx, ty = -mj * cr * dy + dny, tz = -mj * cr * dz + dnz;
-        gs += fij * (ali_x * tx + ali_y * ty + ali_z * tz);
-        nbx += gam * (fij * ali_x - fji * alj_x);
-        nby += gam * (fij * ali_y - fji * alj_y);
-        nbz += gam * (fij * ali_z - fji * alj_z);
-        T wx = fij * mj * ali_x - fji * mi * alj_x;
-        T wy = fij * mj * ali_y - fji * mi * alj_y;
-        T wz = fij * mj * ali_z - fji * mi * alj_z;
-        T s = cq * (dx * wx + dy * wy + dz * wz);
+        coh_d1(k, r2 * rinv, &C, &Cp);
+        const T cr = C * rinv;
+        const T cq = (Cp - cr) * (rinv * rinv);
+        const T alj_x = dt * d.uj.x, alj_y = dt * d.uj.y, alj_z = dt * d.uj.z;
+        const T mj = q.w;  // DFSPH: the alias X is the mass
+        const T rden = f_rcp(rhoi + d.nj.w);
+        const T fij = T(2) * r0i * rden, fji = T(2) * d.r0j * rden;
+        const T ad = ali_x * dx + ali_y * dy + ali_z * dz;
+        const T jd = alj_x * dx + alj_y * dy + alj_z * dz;
+        gs += fij * (-mj * cr) * ad;
+        const T wx = fij * mj * ali_x - fji * mi * alj_x;
+        const T wy = fij * mj * ali_y - fji * mi * alj_y;
+        const T wz = fij * mj * ali_z - fji * mi * alj_z;
+        const T s = cq * (dx * wx + dy * wy + dz * wz);
         ax += -gam * (cr * wx + s * dx);
         ay += -gam * (cr * wy + s * dy);
         az += -gam * (cr * wz + s * dz);
-        // s_jl: j's gamma-free force from l is m_l C x/r - (n_l - n_j)
-        T sl = ali_x * tx + ali_y * ty + ali_z * tz;
-        T sj = alj_x * (mi * cr * dx - dnx) + alj_y * (mi * cr * dy - dny) +
-               alj_z * (mi * cr * dz - dnz);
-        rb += -gam * (fij * sl + fji * sj) * rden;
-      });
+        rb += -gam * rden * (fji * mi * cr * jd - fij * mj * cr * ad);
+      };
+      const int nf = nbr_nf[i];
+      if (nf <= k.list_cap_f) {
+        sweep_one_list_coop<1, CandC, 2>(cap, i, 0, nbr_f, nf, loadc, coh);
+      } else {
+        walk_cells(k, fstart, c, [&](int j) {
+          const V4<T> q = fpos[j];
+          const T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
+          if (r2 > T(0) && r2 <= k.r2max) coh(loadc(j));
+        });
+      }
     }
     nbar[i] = v4(nbx, nby, nbz);
     rhobar[i] += rb;
@@ -717,8 +748,9 @@ int adj_advect(const TsphParams& prm, TsphObject* f, TsphControl* ctrl, TsphAdjo
   // of the surface-tension reverse vanishes and it is skipped
   if (a->nbar) {
     hipLaunchKernelGGL(k_adj_surface_tension<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n,
-                       (const V4<T>*)f->pos, (const T*)f->mass, (const T*)f->sph_density,
-                       (const T*)f->rest_density, (const T*)f->volume_frac, f->cell_start,
+                       f->capacity, (const V4<T>*)f->pos, (const T*)f->mass,
+                       (const T*)f->sph_density, (const T*)f->rest_density,
+                       (const T*)f->volume_frac, f->cell_start, f->nbr_f, f->nbr_nf,
                        (const V4<T>*)a->tape_normal, (const V4<T>*)a->vbar, (V4<T>*)a->nbar,
                        (T*)a->rhobar, (V4<T>*)a->xbar, L.partial, ctrl);
     hipLaunchKernelGGL(k_adj_grad_add, dim3(1), dim3(kBlock), 0, st, L.partial, nb, a->grad, 1);

@@ -676,19 +676,25 @@ __global__ void __launch_bounds__(kBlock)
 
 // Surface tension loop 2 (sph.py:151-166) + SPH_advection_update_vel_adv
 // (sph.py:247-250).  The curvature term carries no kernel weight, so every
-// candidate of the neighbouring cells contributes: this sweep walks the cells.
-// With gamma == 0 every term is an exact zero and only the update runs.
-// Candidate records in flight per lane of the forward surface-tension walk
-// (its records are 44 B; the adjoint walk's 64-B records keep kStU).
+// candidate of the neighbouring cells contributes to it: that part walks the
+// cells, gathering only the normal record.  Its r > 0 test is j != i: two
+// distinct particles at one position have bit-equal normals (their neighbour
+// sums run over the same particles in the same order), so their curvature
+// term is an exact zero either way.  The cohesion term's C(r) vanishes
+// beyond h, so it is summed over the in-range list.  Both terms share the
+// factor 2 psi_0 / (psi_i + psi_j) and gamma.  With gamma == 0 every term is
+// an exact zero and only the update runs.
+// Candidate records in flight per lane of the forward curvature walk.
 constexpr int kStFwdU = 8;
 template <class T>
-__device__ __forceinline__ void k_surface_tension_body(Blk blk, const KP<T>& k, int n, const V4<T>* __restrict__ fpos,
-                      const T* __restrict__ psi_rho,
+__device__ __forceinline__ void k_surface_tension_body(Blk blk, const KP<T>& k, int n, int cap,
+                      const V4<T>* __restrict__ fpos, const T* __restrict__ psi_rho,
                       const T* __restrict__ rdens, const T* __restrict__ vf,
                       const int* __restrict__ fstart, const V4<T>* __restrict__ normal,
-                      const V4<T>* __restrict__ strec,
-                      const V4<T>* __restrict__ vel, V4<T>* __restrict__ acce_adv,
-                      V4<T>* __restrict__ vadv, const TsphControl* ctrl) {
+                      const V4<T>* __restrict__ strec, const int* __restrict__ nbr_f,
+                      const int* __restrict__ nbr_nf, const V4<T>* __restrict__ vel,
+                      V4<T>* __restrict__ acce_adv, V4<T>* __restrict__ vadv,
+                      const TsphControl* ctrl) {
   int i = xcd_block(blk.b, blk.nb) * kBlock + threadIdx.x;
   if (i >= n) return;
   const T dt = (T)ctrl->dt;
@@ -700,27 +706,51 @@ __device__ __forceinline__ void k_surface_tension_body(Blk blk, const KP<T>& k, 
     T psi_i = psi_rho[i];
     int c[3];
     node_encode(k, p.x, p.y, p.z, c);
-    // per candidate: (x_j, m_j) and (n_j, +-psi_j), written by the advection
-    // sweep (psi_j < 0 where j's 0.99 gate fails)
-    auto load = [&](int j) { return Rec2<T>{strec[j], normal[j]}; };
-    walk_cells_prefetch<kStFwdU>(k, fstart, c, load, [&](int, const Rec2<T>& d) {
-      // every candidate of the 27 cells: reciprocal square root and
-      // reciprocal instead of the IEEE square root and divisions (the pair
-      // terms of this walk are not part of the bit-exact sums)
-      const V4<T>& q = d.a;
-      const T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-      const T r2 = dx * dx + dy * dy + dz * dz;
-      if (r2 > T(0) && d.b.w > T(0)) {
-        const T rinv = f_rsqrt(r2);
-        const T cs = -k.gamma * q.w * f_coh(k, r2 * rinv * k.inv_h);
-        const V4<T>& nj = d.b;
-        const T f = T(2) * rp * f_rcp(psi_i + nj.w);
-        const T csr = cs * rinv;
-        a.x += f * (csr * dx + k.gamma * (ni.x - nj.x));
-        a.y += f * (csr * dy + k.gamma * (ni.y - nj.y));
-        a.z += f * (csr * dz + k.gamma * (ni.z - nj.z));
+    // the normal record's spare lane holds +-psi_j, negative where j's 0.99
+    // gate fails (the advection sweep writes it).  Reciprocal square root
+    // and reciprocals instead of the IEEE forms: the pair terms of these
+    // sums are not part of the bit-exact sums.
+    T ux = 0, uy = 0, uz = 0;  // sum of (n_i - n_j) / (psi_i + psi_j)
+    walk_cells_prefetch<kStFwdU>(k, fstart, c, [&](int j) { return normal[j]; },
+                                 [&](int j, const V4<T>& nj) {
+      if (j != i && nj.w > T(0)) {
+        const T w = f_rcp(psi_i + nj.w);
+        ux += w * (ni.x - nj.x);
+        uy += w * (ni.y - nj.y);
+        uz += w * (ni.z - nj.z);
       }
     });
+    // per in-range neighbour: (x_j, m_j) and the normal record
+    auto load = [&](int j) { return Rec2<T>{strec[j], normal[j]}; };
+    T hx = 0, hy = 0, hz = 0;  // sum of m_j C(r) x_ij / r / (psi_i + psi_j)
+    auto coh = [&](const Rec2<T>& d) {
+      if (d.b.w > T(0)) {
+        const V4<T>& q = d.a;
+        const T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+        const T r2 = dx * dx + dy * dy + dz * dz;
+        const T rinv = f_rsqrt(r2);
+        const T s = q.w * f_coh(k, r2 * rinv * k.inv_h) * rinv * f_rcp(psi_i + d.b.w);
+        hx += s * dx;
+        hy += s * dy;
+        hz += s * dz;
+      }
+    };
+    const int nf = nbr_nf[i];
+    if (nf <= k.list_cap_f) {
+      sweep_one_list<Rec2<T>>(cap, i, nbr_f, nf, load, coh);
+    } else {
+      walk_cells(k, fstart, c, [&](int j) {
+        const V4<T> q = fpos[j];
+        const T r2 = s_r2(p.x - q.x, p.y - q.y, p.z - q.z);
+        if (r2 > T(0) && r2 <= k.r2max) coh(load(j));
+      });
+    }
+    // 2 psi_0 / (psi_i + psi_j) (cohesion + curvature), cohesion =
+    // -gamma m_j C(r) x_ij / r, curvature = gamma (n_i - n_j)
+    const T f = T(2) * rp * k.gamma;
+    a.x += f * (ux - hx);
+    a.y += f * (uy - hy);
+    a.z += f * (uz - hz);
     acce_adv[i] = a;
   }
   {
@@ -747,14 +777,15 @@ __global__ void __launch_bounds__(kBlock)
 
 template <class T>
 __global__ void __launch_bounds__(kBlock)
-    k_surface_tension(KP<T> k, int n, const V4<T>* __restrict__ fpos,
+    k_surface_tension(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
                       const T* __restrict__ psi_rho,
                       const T* __restrict__ rdens, const T* __restrict__ vf,
                       const int* __restrict__ fstart, const V4<T>* __restrict__ normal,
-                      const V4<T>* __restrict__ strec,
-                      const V4<T>* __restrict__ vel, V4<T>* __restrict__ acce_adv,
-                      V4<T>* __restrict__ vadv, const TsphControl* ctrl) {
-  k_surface_tension_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, n, fpos, psi_rho, rdens, vf, fstart, normal, strec, vel, acce_adv, vadv, ctrl);
+                      const V4<T>* __restrict__ strec, const int* __restrict__ nbr_f,
+                      const int* __restrict__ nbr_nf, const V4<T>* __restrict__ vel,
+                      V4<T>* __restrict__ acce_adv, V4<T>* __restrict__ vadv,
+                      const TsphControl* ctrl) {
+  k_surface_tension_body<T>(Blk{(int)blockIdx.x, (int)gridDim.x}, k, n, cap, fpos, psi_rho, rdens, vf, fstart, normal, strec, nbr_f, nbr_nf, vel, acce_adv, vadv, ctrl);
 }
 
 // ---------------------------------------------------------------------------
@@ -1203,10 +1234,10 @@ int advect(const TsphParams& prm, TsphObject* f, TsphObject* b, TsphControl* ctr
                        (const V4<T>*)f->vel, (const V4<T>*)f->acce_adv, (V4<T>*)f->vel_adv, ctrl);
   else
     hipLaunchKernelGGL(k_surface_tension<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n,
-                       (const V4<T>*)f->pos, psi, (const T*)f->rest_density,
+                       f->capacity, (const V4<T>*)f->pos, psi, (const T*)f->rest_density,
                        (const T*)f->volume_frac, f->cell_start, (const V4<T>*)f->normal,
-                       (const V4<T>*)L.krec, (const V4<T>*)f->vel, (V4<T>*)f->acce_adv,
-                       (V4<T>*)f->vel_adv, ctrl);
+                       (const V4<T>*)L.krec, f->nbr_f, f->nbr_nf, (const V4<T>*)f->vel,
+                       (V4<T>*)f->acce_adv, (V4<T>*)f->vel_adv, ctrl);
   TSPH_LAUNCH_CHECK("advect");
   return 0;
 }
@@ -1532,10 +1563,10 @@ template <class T>
 __global__ void __launch_bounds__(kBlock) kb_surface_tension(const char* __restrict__ tab, int ns) {
   for_tiles<T, kTileFluid>(tab, ns, [&](const SceneEntry<T>& e, int, Blk blk) {
     const TsphObject& f = e.f;
-    k_surface_tension_body<T>(blk, e.k, f.n, (const V4<T>*)f.pos, psi_of(e),
+    k_surface_tension_body<T>(blk, e.k, f.n, f.capacity, (const V4<T>*)f.pos, psi_of(e),
                               (const T*)f.rest_density, (const T*)f.volume_frac, f.cell_start,
-                              (const V4<T>*)f.normal, (const V4<T>*)e.L.krec, (const V4<T>*)f.vel,
-                              (V4<T>*)f.acce_adv, (V4<T>*)f.vel_adv, e.ctrl);
+                              (const V4<T>*)f.normal, (const V4<T>*)e.L.krec, f.nbr_f, f.nbr_nf,
+                              (const V4<T>*)f.vel, (V4<T>*)f.acce_adv, (V4<T>*)f.vel_adv, e.ctrl);
   });
 }
 
