@@ -7,7 +7,14 @@ Finite differences (tests/test_gpu_adjoint.py) check one direction at a
 time; here every component of d loss / d (x0, v0) and both parameter
 derivatives are compared.  The loss is a random linear functional of the
 final positions and velocities, so its gradient is the vector-Jacobian
-product of the three-step map itself."""
+product of the three-step map itself.
+
+The coincident case puts two fluid particles at one position.  Every pair
+term of the reference requires r > 0, so that pair contributes nothing; the
+surface-tension sums test r > 0 through j != i and check positions only in
+the particle's own cell, and the normal-adjoint term of such a pair would not
+vanish if they did not.  Autograd through the dense step, which masks r = 0
+pairs, pins that handling."""
 import numpy as np
 import pytest
 import torch
@@ -24,19 +31,29 @@ def rel_vec(a, b):
     return float((a - b).abs().max() / b.abs().max())
 
 
+@pytest.mark.parametrize("coincident", [False, True], ids=["plain", "coincident"])
 @pytest.mark.parametrize("dim", [2, 3])
-def test_adjoint_gradient_equals_autograd_of_dense_step(dim):
+def test_adjoint_gradient_equals_autograd_of_dense_step(dim, coincident):
     from tichi_sph_amd import simulation_from_dicts
     from tichi_sph_amd.adjoint import DiffRunner
     cb, sb = tiny_dambreak(dim, gamma=GAMMA)
-    print(f"[case] tiny {dim}D dam break, {STEPS} steps, adjoint vs dense autograd", flush=True)
+    print(f"[case] tiny {dim}D dam break{' with a coincident pair' if coincident else ''}, {STEPS} steps, "
+          f"adjoint vs dense autograd", flush=True)
     sim = simulation_from_dicts(cb, sb, precision=8, cfl=False)
     r = DiffRunner(sim)
     f = sim.fluid
     n, nb = f.part_num, sim.bound.part_num
     dev = sim.device
+    if coincident:
+        # the particle after the block's middle one moves onto it (the next
+        # step's sort puts both in one cell)
+        pid = f.t["pid"][:n].long()
+        ia, ib = (int((pid == q).nonzero()[0]) for q in (n // 2, n // 2 + 1))
+        f.t["pos"][ib, :dim] = f.t["pos"][ia, :dim]
     st0 = r.checkpoint()
     x0 = r.positions_ref().detach().cpu()
+    if coincident:
+        assert torch.equal(x0[n // 2], x0[n // 2 + 1])
     # the state's constants at the start (the tail's SPH_update_mass rewrites the masses)
     state = {k: np.asarray(f.to_numpy(k), dtype=np.float64) for k in ("mass", "rest_density", "rest_volume")}
     bound = {k: np.asarray(sim.bound.to_numpy(k), dtype=np.float64) for k in ("pos", "mass")}
