@@ -369,7 +369,21 @@ __global__ void __launch_bounds__(kBlock)
 // no position: r > 0 is j != l except for a particle at l's own position,
 // which lies in l's own cell, so only that cell's candidates are tested.
 // The cohesion terms (C = C' = 0 beyond h) are summed over the in-range list.
+// A neighbour's rest density and velocity adjoint enter only as
+// b_j = rho0_j a_j (f_jl a_j = 2 b_j / (rho_l + rho_j)), which k_adj_st_rec
+// writes per particle, so that a candidate costs two 16-byte gathers.
 // ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_adj_st_rec(int n, const T* __restrict__ rdens, const V4<T>* __restrict__ vbar,
+                 V4<T>* __restrict__ brec, const TsphControl* ctrl) {
+  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const T s = rdens[i] * (T)ctrl->dt;
+  const V4<T> u = vbar[i];
+  brec[i] = V4<T>{s * u.x, s * u.y, s * u.z, T(0)};
+}
+
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_adj_surface_tension(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
@@ -377,9 +391,10 @@ __global__ void __launch_bounds__(kBlock)
                           const T* __restrict__ rdens, const T* __restrict__ vf,
                           const int* __restrict__ fstart, const int* __restrict__ nbr_f,
                           const int* __restrict__ nbr_nf, const V4<T>* __restrict__ tnormal,
-                          const V4<T>* __restrict__ vbar, V4<T>* __restrict__ nbar,
-                          T* __restrict__ rhobar, V4<T>* __restrict__ xbar,
-                          double* __restrict__ partial, const TsphControl* ctrl) {
+                          const V4<T>* __restrict__ vbar, const V4<T>* __restrict__ brec,
+                          V4<T>* __restrict__ nbar, T* __restrict__ rhobar,
+                          V4<T>* __restrict__ xbar, double* __restrict__ partial,
+                          const TsphControl* ctrl) {
   __shared__ double lds[kWaves];
   const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   double gsum = 0;
@@ -399,25 +414,23 @@ __global__ void __launch_bounds__(kBlock)
       const int own0 = fstart[own], own1 = fstart[own + 1];
       // curvature, per candidate: the recorded normal with +-rho_j in its
       // spare lane (negative where j's 0.99 gate fails; the forward advection
-      // writes it), vbar_j and rho0_j
-      struct CandN {
-        V4<T> nj, uj;
-        T r0j;
-      };
-      auto loadn = [&](int j) { return CandN{tnormal[j], vbar[j], rdens[j]}; };
-      walk_cells_prefetch<kStU>(k, fstart, c, loadn, [&](int j, const CandN& d) {
-        if (j == i || !(d.nj.w > T(0))) return;
+      // writes it) and b_j
+      auto loadn = [&](int j) { return Rec2<T>{tnormal[j], brec[j]}; };
+      walk_cells_prefetch<kStU>(k, fstart, c, loadn, [&](int j, const Rec2<T>& d) {
+        if (j == i || !(d.a.w > T(0))) return;
         if (j >= own0 && j < own1) {
           const V4<T> q = fpos[j];
           if (!(s_r2(p.x - q.x, p.y - q.y, p.z - q.z) > T(0))) return;
         }
-        const V4<T>& nj = d.nj;
+        const V4<T>& nj = d.a;
+        const V4<T>& bj = d.b;
         const T rden = f_rcp(rhoi + nj.w);
-        const T fij = T(2) * r0i * rden, fji = T(2) * d.r0j * rden;
+        const T fij = T(2) * r0i * rden, r2d = T(2) * rden;
         const T dnx = ni.x - nj.x, dny = ni.y - nj.y, dnz = ni.z - nj.z;
-        const T ex = fij * ali_x - fji * (dt * d.uj.x);
-        const T ey = fij * ali_y - fji * (dt * d.uj.y);
-        const T ez = fij * ali_z - fji * (dt * d.uj.z);
+        // e_lj = f_lj a_l - f_jl a_j
+        const T ex = fij * ali_x - r2d * bj.x;
+        const T ey = fij * ali_y - r2d * bj.y;
+        const T ez = fij * ali_z - r2d * bj.z;
         gs += fij * (ali_x * dnx + ali_y * dny + ali_z * dnz);
         nbx += gam * ex;
         nby += gam * ey;
@@ -425,14 +438,11 @@ __global__ void __launch_bounds__(kBlock)
         rb += -gam * rden * (dnx * ex + dny * ey + dnz * ez);
       });
       // cohesion, per in-range neighbour: (x_j, X_j = m_j) as well
-      struct CandC {
-        V4<T> q, nj, uj;
-        T r0j;
-      };
-      auto loadc = [&](int j) { return CandC{fpos[j], tnormal[j], vbar[j], rdens[j]}; };
-      auto coh = [&](const CandC& d) {
-        if (!(d.nj.w > T(0))) return;
-        const V4<T>& q = d.q;
+      auto loadc = [&](int j) { return Rec3<T>{fpos[j], tnormal[j], brec[j]}; };
+      auto coh = [&](const Rec3<T>& d) {
+        if (!(d.b.w > T(0))) return;
+        const V4<T>& q = d.a;
+        const V4<T>& bj = d.c;
         const T dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
         const T r2 = dx * dx + dy * dy + dz * dz;
         const T rinv = f_rsqrt(r2);
@@ -440,25 +450,25 @@ __global__ void __launch_bounds__(kBlock)
         coh_d1(k, r2 * rinv, &C, &Cp);
         const T cr = C * rinv;
         const T cq = (Cp - cr) * (rinv * rinv);
-        const T alj_x = dt * d.uj.x, alj_y = dt * d.uj.y, alj_z = dt * d.uj.z;
         const T mj = q.w;  // DFSPH: the alias X is the mass
-        const T rden = f_rcp(rhoi + d.nj.w);
-        const T fij = T(2) * r0i * rden, fji = T(2) * d.r0j * rden;
+        const T rden = f_rcp(rhoi + d.b.w);
+        const T fij = T(2) * r0i * rden, r2d = T(2) * rden;
         const T ad = ali_x * dx + ali_y * dy + ali_z * dz;
-        const T jd = alj_x * dx + alj_y * dy + alj_z * dz;
+        const T bd = bj.x * dx + bj.y * dy + bj.z * dz;
         gs += fij * (-mj * cr) * ad;
-        const T wx = fij * mj * ali_x - fji * mi * alj_x;
-        const T wy = fij * mj * ali_y - fji * mi * alj_y;
-        const T wz = fij * mj * ali_z - fji * mi * alj_z;
+        // f_lj m_j a_l - f_jl m_l a_j
+        const T wx = fij * mj * ali_x - r2d * mi * bj.x;
+        const T wy = fij * mj * ali_y - r2d * mi * bj.y;
+        const T wz = fij * mj * ali_z - r2d * mi * bj.z;
         const T s = cq * (dx * wx + dy * wy + dz * wz);
         ax += -gam * (cr * wx + s * dx);
         ay += -gam * (cr * wy + s * dy);
         az += -gam * (cr * wz + s * dz);
-        rb += -gam * rden * (fji * mi * cr * jd - fij * mj * cr * ad);
+        rb += -gam * rden * (r2d * mi * cr * bd - fij * mj * cr * ad);
       };
       const int nf = nbr_nf[i];
       if (nf <= k.list_cap_f) {
-        sweep_one_list_coop<1, CandC, 2>(cap, i, 0, nbr_f, nf, loadc, coh);
+        sweep_one_list_coop<1, Rec3<T>, 2>(cap, i, 0, nbr_f, nf, loadc, coh);
       } else {
         walk_cells(k, fstart, c, [&](int j) {
           const V4<T> q = fpos[j];
@@ -747,11 +757,16 @@ int adj_advect(const TsphParams& prm, TsphObject* f, TsphControl* ctrl, TsphAdjo
   // nbar == NULL: gamma == 0 and its gradient is not wanted, so every term
   // of the surface-tension reverse vanishes and it is skipped
   if (a->nbar) {
+    // b_j = rho0_j dt vbar_j into the workspace's record buffer (free between
+    // forward Jacobi iterations)
+    hipLaunchKernelGGL(k_adj_st_rec<T>, dim3(nb), dim3(kBlock), 0, st, f->n,
+                       (const T*)f->rest_density, (const V4<T>*)a->vbar, (V4<T>*)L.krec, ctrl);
     hipLaunchKernelGGL(k_adj_surface_tension<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n,
                        f->capacity, (const V4<T>*)f->pos, (const T*)f->mass,
                        (const T*)f->sph_density, (const T*)f->rest_density,
                        (const T*)f->volume_frac, f->cell_start, f->nbr_f, f->nbr_nf,
-                       (const V4<T>*)a->tape_normal, (const V4<T>*)a->vbar, (V4<T>*)a->nbar,
+                       (const V4<T>*)a->tape_normal, (const V4<T>*)a->vbar,
+                       (const V4<T>*)L.krec, (V4<T>*)a->nbar,
                        (T*)a->rhobar, (V4<T>*)a->xbar, L.partial, ctrl);
     hipLaunchKernelGGL(k_adj_grad_add, dim3(1), dim3(kBlock), 0, st, L.partial, nb, a->grad, 1);
   }

@@ -182,6 +182,10 @@ template <class T>
 struct Rec2 {
   V4<T> a, b;
 };
+template <class T>
+struct Rec3 {
+  V4<T> a, b, c;
+};
 
 // ---------------------------------------------------------------------------
 // Cooperative list sweeps.  A group of kCoop adjacent lanes shares one
