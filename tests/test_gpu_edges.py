@@ -11,7 +11,8 @@
   solve's mean compression is 0/0 = NaN (sph.py:221-226), so the solve runs
   its minimum iteration count.
 Both are compared with the oracle on the same inputs, to the bar of the
-other parity tests.
+other parity tests.  The adjoint over the stray particles, with surface
+tension on, is compared with central differences of the recorded step.
 """
 import copy
 
@@ -25,7 +26,7 @@ from test_gpu_parity import FIELDS_F, TOL, compare_fields, make, rel_err
 pytestmark = pytest.mark.gpu
 
 
-def _stray_positions(sim, pos):
+def _stray_positions(sim, pos, dtype=np.float32):
     """Moves six interior particles (reference order) to the edge cases."""
     cfg = sim.config
     d = cfg.dim
@@ -40,7 +41,7 @@ def _stray_positions(sim, pos):
     p[idx[2]] = lb + 0.5 * h                 # cell 0: never inserted
     p[idx[3]] = 1.0e9                        # cell index beyond int32
     p[idx[5]] = p[idx[4]]                    # two particles at one position
-    return p.astype(np.float32), idx
+    return p.astype(dtype), idx
 
 
 @pytest.mark.parametrize("name", ["dambreak_3d_8k", "dambreak_2d_8k"])
@@ -120,3 +121,52 @@ def test_empty_fluid_step_matches_oracle():
     worst = compare_fields(sim, o, ["sph_density", "sph_compression", "alpha", "alpha_2", "W"], obj=1,
                            label="no fluid")
     assert max(worst.values()) == 0  # the boundary's sums are strict: bit-exact
+
+
+def test_adjoint_over_stray_particles_matches_fd_fp64():
+    """The reverse over the same stray particles with surface tension on: the
+    particles outside the grid have no neighbours and zero density (their
+    V = X / psi is infinite but enters no pair term), and two share a
+    position.  The gradients are finite and equal central differences."""
+    import torch
+    from tichi_sph_amd.adjoint import rollout
+    from test_gpu_adjoint import FD_TOL, forward_only, loss_of, rel, runner_for, seeded_velocity
+    steps, every = 3, 3
+    r = runner_for("dambreak_3d_8k", 8)
+    f = r.sim.fluid
+    p, idx = _stray_positions(r.sim, f.to_numpy("pos"), np.float64)
+    print(f"  moved particles {idx}; {steps} steps, gamma = 0.07", flush=True)
+    f.from_numpy("pos", p)
+    st0 = r.checkpoint()
+    v0 = seeded_velocity(r)
+    target = forward_only(r, st0, v0, 0.02, 0.05, steps).detach()
+    r.restore(st0)
+    dev = v0.device
+    mu = torch.tensor(0.01, dtype=torch.float64, device=dev, requires_grad=True)
+    gam = torch.tensor(0.07, dtype=torch.float64, device=dev, requires_grad=True)
+    v = v0.clone().requires_grad_(True)
+    x, _ = rollout(r, v, mu, gam, steps, every)
+    loss_of(x, target).backward()
+    log = list(r.iter_log)
+    print(f"  iterations {log}", flush=True)
+    assert bool(torch.isfinite(v.grad).all()) and np.isfinite(float(mu.grad)) and np.isfinite(float(gam.grad))
+
+    def L_at(mu_, gam_, v_):
+        return float(loss_of(forward_only(r, st0, v_, mu_, gam_, steps, force=log), target))
+
+    g = torch.Generator().manual_seed(7)
+    dv = torch.randn(v0.shape, generator=g, dtype=torch.float64).to(dev)
+    checks = []
+    e = 1e-4 * 0.01
+    checks.append(("mu", float(mu.grad), (L_at(0.01 + e, 0.07, v0) - L_at(0.01 - e, 0.07, v0)) / (2 * e)))
+    e = 1e-4 * 0.07
+    checks.append(("gamma", float(gam.grad), (L_at(0.01, 0.07 + e, v0) - L_at(0.01, 0.07 - e, v0)) / (2 * e)))
+    e = 1e-8
+    fd = (L_at(0.01, 0.07, v0 + e * dv) - L_at(0.01, 0.07, v0 - e * dv)) / (2 * e)
+    checks.append(("v0 . dv", float((v.grad * dv).sum()), fd))
+    bad = []
+    for what, ad, fd in checks:
+        print(f"  {what:8s} adjoint={ad:+.10e} fd={fd:+.10e} rel={rel(ad, fd):.2e}", flush=True)
+        if not rel(ad, fd) <= FD_TOL:
+            bad.append(what)
+    assert not bad, bad

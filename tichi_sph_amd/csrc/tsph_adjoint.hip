@@ -410,7 +410,10 @@ __global__ void __launch_bounds__(kBlock)
       const T rhoi = dens[i], r0i = rdens[i], mi = fmass[i];
       int c[3];
       node_encode(k, p.x, p.y, p.z, c);
-      const int own = dim_encode(k, c[0], c[1], c[2]);
+      // the particle's own cell; a code outside (0, node_num) is the sort's
+      // trailing bucket node_num, whose members no cell walk visits
+      const int code = dim_encode(k, c[0], c[1], c[2]);
+      const long own = (0 < code && code < k.node_num) ? (long)code : (long)k.node_num;
       const int own0 = fstart[own], own1 = fstart[own + 1];
       // curvature, per candidate: the recorded normal with +-rho_j in its
       // spare lane (negative where j's 0.99 gate fails; the forward advection
@@ -592,7 +595,9 @@ __global__ void __launch_bounds__(kBlock)
     ms = group_sum<kCoopR>(ms);
     if (cl != 0) continue;
     vbar_out[i] = v4(ui.x + vbx, ui.y + vby, ui.z + vbz, kn);
-    rhobar[i] = rb0 + -Vb * Vi / di;
+    // V_i = X_i / psi_i enters only pair terms; a particle outside the grid
+    // (psi_i = 0, V_i infinite) has none, so there is nothing to carry back
+    rhobar[i] = di > T(0) ? rb0 + -Vb * Vi / di : rb0;
     xbar[i] = v4(xb.x + xbx, xb.y + xby, xb.z + xbz);
     msum += (double)ms;
   }
