@@ -505,6 +505,20 @@ struct Rec4 {
   V4<T> a, b, c, d;
 };
 
+// Per fluid particle: a_bar = dt u and fv = 2 (2 + d) mu / rho0, the record
+// the viscosity reverse gathers per neighbour (one gather instead of u and
+// rho0 apart).
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_adj_vis_rec(KP<T> k, int n, const T* __restrict__ rdens, const V4<T>* __restrict__ vbar,
+                  V4<T>* __restrict__ arec, const TsphControl* ctrl) {
+  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const T dt = (T)ctrl->dt;
+  const V4<T> u = vbar[i];
+  arec[i] = v4(dt * u.x, dt * u.y, dt * u.z, k.two_d2 * k.mu * f_rcp(rdens[i]));
+}
+
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_adj_viscosity(KP<T> k, int n, int cap, const V4<T>* __restrict__ fpos,
@@ -513,7 +527,8 @@ __global__ void __launch_bounds__(kBlock)
                     const int* __restrict__ bstart, const V4<T>* __restrict__ bpos,
                     const int* __restrict__ nbr_f, const int* __restrict__ nbr_nf,
                     const V4<T>* __restrict__ tvel, const V4<T>* __restrict__ nbar,
-                    const V4<T>* __restrict__ vbar, const T* __restrict__ kappa_next,
+                    const V4<T>* __restrict__ vbar, const V4<T>* __restrict__ arec,
+                    const T* __restrict__ kappa_next,
                     V4<T>* __restrict__ vbar_out, T* __restrict__ rhobar,
                     V4<T>* __restrict__ xbar, double* __restrict__ partial,
                     const TsphControl* ctrl) {
@@ -579,10 +594,8 @@ __global__ void __launch_bounds__(kBlock)
     sweep_lists_coop<kCoopR, T, Rec4<T>, kNbVis>(
         k, cap, i, cl, p, fstart, bstart, fpos, bpos, nbr_f, nullptr, nbr_nf[i], 0,
         [&](int j) {
-          const V4<T> v = tvel[j];  // w = V_j
-          V4<T> u = vbar[j];
-          V4<T> a = v4(dt * u.x, dt * u.y, dt * u.z, k.two_d2 * k.mu * f_rcp(rdens[j]));
-          return Rec4<T>{fpos[j], v, a, nbar ? nbar[j] : V4<T>{}};
+          // tvel.w = V_j; arec = (dt u_j, fv_j)
+          return Rec4<T>{fpos[j], tvel[j], arec[j], nbar ? nbar[j] : V4<T>{}};
         },
         [&](int j) { return Rec4<T>{}; }, term, [&](const Rec4<T>&) {});
     vbx = group_sum<kCoopR>(vbx);
@@ -775,11 +788,16 @@ int adj_advect(const TsphParams& prm, TsphObject* f, TsphControl* ctrl, TsphAdjo
                        (T*)a->rhobar, (V4<T>*)a->xbar, L.partial, ctrl);
     hipLaunchKernelGGL(k_adj_grad_add, dim3(1), dim3(kBlock), 0, st, L.partial, nb, a->grad, 1);
   }
+  // (dt vbar, fv) per particle into the workspace's boundary alpha_1 buffer
+  // (pass A's output for the forward psi sweeps, free during a reverse)
+  hipLaunchKernelGGL(k_adj_vis_rec<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n,
+                     (const T*)f->rest_density, (const V4<T>*)a->vbar, (V4<T>*)L.a1b, ctrl);
   hipLaunchKernelGGL(k_adj_viscosity<T>, dim3(nb), dim3(kBlock), 0, st, k, f->n, f->capacity,
                      (const V4<T>*)f->pos, (const T*)f->sph_density,
                      (const T*)f->rest_density, f->cell_start, (const int*)nullptr,
                      (const V4<T>*)nullptr, f->nbr_f, f->nbr_nf, (const V4<T>*)a->tape_vel,
-                     (const V4<T>*)a->nbar, (const V4<T>*)a->vbar, (const T*)a->kappa_next,
+                     (const V4<T>*)a->nbar, (const V4<T>*)a->vbar, (const V4<T>*)L.a1b,
+                     (const T*)a->kappa_next,
                      (V4<T>*)a->vbar_out, (T*)a->rhobar, (V4<T>*)a->xbar, L.partial, ctrl);
   hipLaunchKernelGGL(k_adj_grad_add, dim3(1), dim3(kBlock), 0, st, L.partial, nb, a->grad, 0);
   TSPH_LAUNCH_CHECK("adj_advect");
