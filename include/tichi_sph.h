@@ -158,7 +158,13 @@ size_t tsph_control_bytes(void);
  */
 int tsph_verify_fast_quotient(const TsphParams* p, int32_t* mismatches, void* stream);
 
-/* Workspace needed by the calls below for an object of `capacity` slots. */
+/* Workspace needed by the calls below for an object of `capacity` slots.
+ * It carries records between the calls of one step: tsph_density_alpha
+ * writes the boundary neighbours' share of alpha_1 that the step's psi
+ * half-steps read, and each psi half-step writes the (x, kappa) records its
+ * update half-step gathers.  Pass one workspace to every call of a step and
+ * run nothing else on it in between (tsph_adj_advect uses both record
+ * regions as scratch). */
 size_t tsph_workspace_bytes(const TsphParams* p, int32_t capacity);
 
 /* ----------------------------------------------------------------------
@@ -434,7 +440,9 @@ int tsph_adj_solver_iteration(const TsphParams* p, TsphObject* fluid, TsphObject
  * vel_adv = vel + dt * acce_adv; sph.py:108-166, 247-250): vbar -> vbar_out,
  * accumulating xbar, rhobar and adj->grad.  adj->nbar may be NULL when
  * surface_tension_gamma == 0 and d loss / d gamma is not wanted: every
- * surface-tension term then vanishes and that reverse sweep is skipped. */
+ * surface-tension term then vanishes and that reverse sweep is skipped.
+ * The workspace's record regions hold per-particle records of the reverse
+ * sweeps during the call (see tsph_workspace_bytes). */
 int tsph_adj_advect(const TsphParams* p, TsphObject* fluid, TsphControl* ctrl, TsphAdjoint* adj,
                     void* workspace, size_t ws_bytes, void* stream);
 
