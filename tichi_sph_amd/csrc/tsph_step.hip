@@ -61,6 +61,10 @@ struct PassAAcc {
   }
 };
 
+// List slots per particle staged in LDS by pass A (kPassAK x kBlock ints per
+// block; 20 KB keeps the kernel at its register-bound occupancy).
+constexpr int kPassAK = 20;
+
 // ---------------------------------------------------------------------------
 // Pass A, fluid particles: SPH_prepare_attr (f,f)(f,b), SPH_prepare_alpha_1
 // (f,f)(f,b), SPH_prepare_alpha_2 (f,f), SPH_prepare_alpha, SPH_vel_2_vel_adv,
@@ -113,10 +117,13 @@ __device__ __forceinline__ void k_pass_a_fluid_body(Blk blk, const KP<T>& k, int
                    T* __restrict__ alpha, T* __restrict__ vol, const V4<T>* __restrict__ vel,
                    V4<T>* __restrict__ vel_adv, int* __restrict__ nbr_f, int* __restrict__ nbr_b,
                    int* __restrict__ nbr_nf, int* __restrict__ nbr_nb, int* __restrict__ flag,
-                   V4<T>* __restrict__ a1b, TsphControl* ctrl) {
+                   V4<T>* __restrict__ a1b, TsphControl* ctrl, int* __restrict__ stage) {
   const int i = xcd_block(blk.b, blk.nb) * kBlock + threadIdx.x;
   if (i >= n) return;
   const int capf = k.list_cap_f, capb = k.list_cap_b;
+  // the first kPassAK list slots go to this lane's column of the block's
+  // LDS stage; the rest are appended to the list directly
+  const int nstage = capf < kPassAK ? capf : kPassAK;
   flag[i] = 0;  // SPH_clean_value, sph.py:30
   const V4<T> p = fpos[i];
   const T mi = fmass[i];
@@ -132,7 +139,10 @@ __device__ __forceinline__ void k_pass_a_fluid_body(Blk blk, const KP<T>& k, int
     if (s_r2(p.x - q.x, p.y - q.y, p.z - q.z) <= k.r2max) {
       const T o = fo[j];
       if (acc.template add<FQ, XM>(k, p, q, XM ? q.w : o, XM ? o : q.w, true)) {
-        if (nf < capf) nbr_f[(long)nf * cap + i] = j;
+        if (nf < nstage)
+          stage[nf * kBlock + threadIdx.x] = j;
+        else if (nf < capf)
+          nbr_f[(long)nf * cap + i] = j;
         ++nf;
       }
     }
@@ -146,6 +156,11 @@ __device__ __forceinline__ void k_pass_a_fluid_body(Blk blk, const KP<T>& k, int
       }
     }
   });
+  // the staged slots, row by row: the wave's lanes store consecutive
+  // entries of one slot row together, whole lines at a time, instead of one
+  // scattered entry per lane whenever a lane finds a neighbour
+  const int ns = nf < nstage ? nf : nstage;
+  for (int s = 0; s < ns; ++s) nbr_f[(long)s * cap + i] = stage[s * kBlock + threadIdx.x];
   a1b[i] = V4<T>{acc.bx, acc.by, acc.bz, T(0)};
   pass_a_epilogue(k, i, p, mi, acc, nf, nbb, W, comp, dens, alpha_1, alpha_2, alpha, vol, vel,
                   vel_adv, nbr_nf, nbr_nb, ctrl);
@@ -179,9 +194,10 @@ __global__ void __launch_bounds__(kBlock)
                    V4<T>* __restrict__ vel_adv, int* __restrict__ nbr_f, int* __restrict__ nbr_b,
                    int* __restrict__ nbr_nf, int* __restrict__ nbr_nb, int* __restrict__ flag,
                    V4<T>* __restrict__ a1b, TsphControl* ctrl) {
+  __shared__ int stage[kPassAK * kBlock];
   const Blk blk{(int)blockIdx.x, (int)gridDim.x};
   pass_a_fluid_variant(k, [&](auto fq, auto xm) {
-    k_pass_a_fluid_body<T, decltype(fq)::value, decltype(xm)::value>(blk, k, n, cap, fpos, fmass, frvol, fstart, nb, bpos, bmass, brvol, bstart, W, comp, dens, alpha_1, alpha_2, alpha, vol, vel, vel_adv, nbr_f, nbr_b, nbr_nf, nbr_nb, flag, a1b, ctrl);
+    k_pass_a_fluid_body<T, decltype(fq)::value, decltype(xm)::value>(blk, k, n, cap, fpos, fmass, frvol, fstart, nb, bpos, bmass, brvol, bstart, W, comp, dens, alpha_1, alpha_2, alpha, vol, vel, vel_adv, nbr_f, nbr_b, nbr_nf, nbr_nb, flag, a1b, ctrl, stage);
   });
 }
 
@@ -283,7 +299,7 @@ __global__ void k_step_reset(TsphControl* ctrl) { step_reset_body(ctrl); }
 template <class T>
 __device__ __forceinline__ void pass_a_both_body(const KP<T>& k, const TsphObject& f,
                                                  const TsphObject& b, int nbf, V4<T>* a1b,
-                                                 TsphControl* ctrl) {
+                                                 TsphControl* ctrl, int* stage) {
   if ((int)blockIdx.x < nbf) {
     pass_a_fluid_variant(k, [&](auto fq, auto xm) {
       k_pass_a_fluid_body<T, decltype(fq)::value, decltype(xm)::value>(
@@ -292,7 +308,7 @@ __device__ __forceinline__ void pass_a_both_body(const KP<T>& k, const TsphObjec
           (const T*)b.rest_volume, b.cell_start, (T*)f.W, (T*)f.sph_compression,
           (T*)f.sph_density, (V4<T>*)f.alpha_1, (T*)f.alpha_2, (T*)f.alpha, (T*)f.vol,
           (const V4<T>*)f.vel, (V4<T>*)f.vel_adv, f.nbr_f, f.nbr_b, f.nbr_nf, f.nbr_nb, f.flag,
-          a1b, ctrl);
+          a1b, ctrl, stage);
     });
   } else {
     k_pass_a_bound_body<T, false>(Blk{(int)blockIdx.x - nbf, (int)gridDim.x - nbf}, k, b.n,
@@ -307,7 +323,8 @@ __device__ __forceinline__ void pass_a_both_body(const KP<T>& k, const TsphObjec
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_pass_a_both(KP<T> k, TsphObject f, TsphObject b, int nbf, V4<T>* a1b, TsphControl* ctrl) {
-  pass_a_both_body<T>(k, f, b, nbf, a1b, ctrl);
+  __shared__ int stage[kPassAK * kBlock];
+  pass_a_both_body<T>(k, f, b, nbf, a1b, ctrl, stage);
 }
 
 // The check behind TSPH_FLAG_FAST_QUOTIENT: for every FP32 r^2 in
@@ -1468,7 +1485,7 @@ __global__ void kb_mark_static_sums(char* tab, int ns) {
 
 // Pass A of one fluid tile / one boundary tile of a scene.
 template <class T>
-__device__ __forceinline__ void pass_a_fluid_tile(const SceneEntry<T>& e, Blk blk) {
+__device__ __forceinline__ void pass_a_fluid_tile(const SceneEntry<T>& e, Blk blk, int* stage) {
   const TsphObject& f = e.f;
   const TsphObject& b = e.b;
   pass_a_fluid_variant(e.k, [&](auto fq, auto xm) {
@@ -1478,7 +1495,7 @@ __device__ __forceinline__ void pass_a_fluid_tile(const SceneEntry<T>& e, Blk bl
         (const T*)b.rest_volume, b.cell_start, (T*)f.W, (T*)f.sph_compression,
         (T*)f.sph_density, (V4<T>*)f.alpha_1, (T*)f.alpha_2, (T*)f.alpha, (T*)f.vol,
         (const V4<T>*)f.vel, (V4<T>*)f.vel_adv, f.nbr_f, f.nbr_b, f.nbr_nf, f.nbr_nb, f.flag,
-        (V4<T>*)e.L.a1b, e.ctrl);
+        (V4<T>*)e.L.a1b, e.ctrl, stage);
   });
 }
 template <class T>
@@ -1499,12 +1516,13 @@ __device__ __forceinline__ void pass_a_bound_tile(const SceneEntry<T>& e, Blk bl
 // the tail of the fluid tiles' launch instead of waiting for it.
 template <class T>
 __global__ void __launch_bounds__(kBlock) kb_pass_a_both(const char* __restrict__ tab, int ns) {
+  __shared__ int stage[kPassAK * kBlock];
   const int tf = tile_prefix<T, kTileFluid>(tab, ns)[ns];
   const int total = tf + tile_prefix<T, kTileBound>(tab, ns)[ns];
   for (int t = blockIdx.x; t < total; t += gridDim.x) {
     if (t < tf)
       on_tile<T, kTileFluid>(tab, ns, t, [&](const SceneEntry<T>& e, int, Blk blk) {
-        pass_a_fluid_tile(e, blk);
+        pass_a_fluid_tile(e, blk, stage);
       });
     else
       on_tile<T, kTileBound>(tab, ns, t - tf, [&](const SceneEntry<T>& e, int, Blk blk) {
