@@ -62,8 +62,8 @@ struct PassAAcc {
 };
 
 // List slots per particle staged in LDS by pass A (kPassAK x kBlock ints per
-// block; 20 KB keeps the kernel at its register-bound occupancy).
-constexpr int kPassAK = 20;
+// block; 22 KB keeps the kernel at its register-bound occupancy).
+constexpr int kPassAK = 22;
 
 // ---------------------------------------------------------------------------
 // Pass A, fluid particles: SPH_prepare_attr (f,f)(f,b), SPH_prepare_alpha_1
@@ -121,9 +121,10 @@ __device__ __forceinline__ void k_pass_a_fluid_body(Blk blk, const KP<T>& k, int
   const int i = xcd_block(blk.b, blk.nb) * kBlock + threadIdx.x;
   if (i >= n) return;
   const int capf = k.list_cap_f, capb = k.list_cap_b;
-  // the first kPassAK list slots go to this lane's column of the block's
-  // LDS stage; the rest are appended to the list directly
+  // the first kPassAK slots of each list go to this lane's column of the
+  // block's LDS stage; the rest are appended to the list directly
   const int nstage = capf < kPassAK ? capf : kPassAK;
+  const int nstage_b = capb < kPassAK ? capb : kPassAK;
   flag[i] = 0;  // SPH_clean_value, sph.py:30
   const V4<T> p = fpos[i];
   const T mi = fmass[i];
@@ -147,20 +148,25 @@ __device__ __forceinline__ void k_pass_a_fluid_body(Blk blk, const KP<T>& k, int
       }
     }
   });
-  walk_cells_batched<kPassAU>(k, bstart, c, bpos, [&](int j, const V4<T>& q) {
-    if (s_r2(p.x - q.x, p.y - q.y, p.z - q.z) <= k.r2max) {
-      const T o = bo[j];
-      if (acc.template add<FQ, XM>(k, p, q, XM ? q.w : o, XM ? o : q.w, false)) {
-        if (nbb < capb) nbr_b[(long)nbb * cap + i] = j;
-        ++nbb;
-      }
-    }
-  });
   // the staged slots, row by row: the wave's lanes store consecutive
   // entries of one slot row together, whole lines at a time, instead of one
   // scattered entry per lane whenever a lane finds a neighbour
   const int ns = nf < nstage ? nf : nstage;
   for (int s = 0; s < ns; ++s) nbr_f[(long)s * cap + i] = stage[s * kBlock + threadIdx.x];
+  walk_cells_batched<kPassAU>(k, bstart, c, bpos, [&](int j, const V4<T>& q) {
+    if (s_r2(p.x - q.x, p.y - q.y, p.z - q.z) <= k.r2max) {
+      const T o = bo[j];
+      if (acc.template add<FQ, XM>(k, p, q, XM ? q.w : o, XM ? o : q.w, false)) {
+        if (nbb < nstage_b)
+          stage[nbb * kBlock + threadIdx.x] = j;
+        else if (nbb < capb)
+          nbr_b[(long)nbb * cap + i] = j;
+        ++nbb;
+      }
+    }
+  });
+  const int nsb = nbb < nstage_b ? nbb : nstage_b;
+  for (int s = 0; s < nsb; ++s) nbr_b[(long)s * cap + i] = stage[s * kBlock + threadIdx.x];
   a1b[i] = V4<T>{acc.bx, acc.by, acc.bz, T(0)};
   pass_a_epilogue(k, i, p, mi, acc, nf, nbb, W, comp, dens, alpha_1, alpha_2, alpha, vol, vel,
                   vel_adv, nbr_nf, nbr_nb, ctrl);
